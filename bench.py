@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused Allen-Cahn explicit step on one MI355X.
+
+Workload (default): 256x256x64 grid, K = 20 orientations, 500 Voronoi grains, cubic
+anisotropy 0.15, moving Gaussian laser, FP32 state and arithmetic.  One step = one
+launch of the fused step kernel over the whole grid (eta already in HBM).
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--quick]
+
+Prints ONE JSON line on stdout (progress goes to stderr).  Metric: cell-orientation
+updates per second (N*K per step).  ``roofline`` prices the step kernel against
+MI355X HBM (8 TB/s) with 8 algorithmic bytes per FP32 update (read + write eta),
+averaging the kernel's duration with HIP events recorded on the stream the kernel runs
+on.  ``cpu_baseline`` times the C/OpenMP FP64 restatement of the reference step on
+this host's cores for a bounded number of steps of the same grid.  Extra keys report
+BF16 storage, the FP64 mode, the 1000-step error of FP32/BF16 against the FP64 mode,
+and (unless --quick) the batched and serving paths.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "jax-am_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = {"float32": 8.0, "bfloat16": 4.0, "float64": 16.0}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_steps(solver, eta0, laser, ts, warmup, steps, tile=None, zchunk=None):
+    """Run `warmup` untimed steps then time exactly `steps` steps. Returns (wall_s, event_ms, final)."""
+    import torch
+    tile = solver.tile if tile is None else tile
+    zchunk = solver.zchunk if zchunk is None else zchunk
+    a = eta0.clone()
+    b = torch.empty_like(a)
+    i = 0
+    for _ in range(warmup):
+        solver.state_rhs.step(a, b, laser, ts[i], ts[i + 1], tile, zchunk)
+        a, b = b, a
+        i += 1
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        solver.state_rhs.step(a, b, laser, ts[i], ts[i + 1], tile, zchunk)
+        a, b = b, a
+        i += 1
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    return wall, e0.elapsed_time(e1), a
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(wl, seconds):
+    """C/OpenMP FP64 restatement (oracle/pf_oracle.c) on this host, bounded to ~`seconds`."""
+    import numpy as np
+    from oracle import c_oracle as C
+    cores = len(os.sched_getaffinity(0))
+    if not C.available():
+        return None
+    P = wl.pf_args
+    params = C.make_params(P, wl.polycrystal.mesh_h_xyz, wl.laser.as_dict())
+    eta = np.zeros((wl.N, wl.K))
+    eta[np.arange(wl.N), wl.polycrystal.cell_ori_inds] = 1.0
+    out = np.empty_like(eta)
+    axes = wl.polycrystal.unique_grain_directions
+    ts = wl.ts(10000)
+    C.step(eta, out, params, axes, ts[0], ts[1], threads=cores)   # warm-up (page-in, thread pool)
+    eta, out = out, eta
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        C.step(eta, out, params, axes, ts[n + 1], ts[n + 2], threads=cores)
+        eta, out = out, eta
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 200:
+            break
+    return dict(value=wl.N * wl.K * n / el, unit="cell*orientation updates/s", cores=cores, kind="port",
+                sample=f"{wl.polycrystal.Nz}x{wl.polycrystal.Ny}x{wl.polycrystal.Nx}x{wl.K} grid, FP64, {n} "
+                       f"steps after 1 warm-up step, {el:.1f} s, OpenMP over {cores} threads "
+                       f"(oracle/pf_oracle.c, {os.uname().machine})")
+
+
+def accuracy(wl, steps):
+    """FP32 and BF16 vs the FP64 mode after `steps` steps from the one-hot initial state."""
+    import numpy as np
+    import torch
+    from pfkit import PFSolver
+    ts = wl.ts(steps)
+    res = {}
+    finals = {}
+    for dt in ("float64", "float32", "bfloat16"):
+        s = PFSolver(wl.pf_args, wl.polycrystal, dtype=dt, clean=False)
+        finals[dt] = (s, s.run(s.ini_cond(), ts, wl.laser))
+    s64, ref = finals["float64"]
+    am_ref = s64.argmax(ref)
+    vol_ref = s64.diagnostics(ref, wl.laser, float(ts[-1]))["volumes"]
+    for dt in ("float32", "bfloat16"):
+        s, e = finals[dt]
+        d = (e.double() - ref).abs().max().item()
+        vol = s.diagnostics(e, wl.laser, float(ts[-1]))["volumes"]
+        rel = np.abs(vol - vol_ref) / np.maximum(np.abs(vol_ref), 1e-300)
+        agree = (s.argmax(e) == am_ref).double().mean().item()
+        res[dt] = dict(steps=steps, max_abs_deta=d, max_rel_volume_err=float(rel.max()),
+                       argmax_agreement=agree)
+    torch.cuda.synchronize()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--quick", action="store_true", help="step throughput + roofline + CPU baseline only")
+    ap.add_argument("--only-step", action="store_true", help="only the timed FP32 steps (for profilers)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--tile", type=int, default=None)
+    ap.add_argument("--zchunk", type=int, default=None)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the phase-field step runs on a single MI355X here.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pfkit import PFSolver, workloads
+    torch.cuda.set_device(0)
+    log("building workload c1 (256x256x64x20)...")
+    wl = workloads.c1()
+    s = PFSolver(wl.pf_args, wl.polycrystal, dtype="float32", clean=False, tile=args.tile, zchunk=args.zchunk)
+    eta0 = s.ini_cond()
+    ts = wl.ts(args.warmup + args.steps)
+    log(f"timing {args.steps} FP32 steps after {args.warmup} warm-up steps (tile {s.tile}, zchunk {s.zchunk})")
+    wall, ev_ms, _ = time_steps(s, eta0, wl.laser, ts, args.warmup, args.steps)
+    upd = wl.N * wl.K
+    value = upd * args.steps / wall
+    kern_s = ev_ms / 1e3 / args.steps
+    achieved = BYTES_PER_UPDATE["float32"] * upd / kern_s / 1e9
+    tr = load_traffic("c1_f32")
+    out = {
+        "metric": "cell*orientation updates/s (fused Allen-Cahn explicit step)",
+        "value": value,
+        "unit": "cell*orientation updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded Voronoi polycrystal, one-hot start, moving Gaussian laser)",
+        "config": {"workload": "c1: 256x256x64 grid, K=20, 500 grains, cubic anisotropy 0.15, FP32",
+                   "cells": wl.N, "orientations": wl.K, "tile": s.tile, "zchunk": s.zchunk,
+                   "dt_us": wl.dt, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (tr or {}).get("bytes_per_launch"),
+                     "kernel_ms": kern_s * 1e3,
+                     "algorithmic_bytes_per_launch": BYTES_PER_UPDATE["float32"] * upd},
+    }
+    if not args.only_step:
+        log("BF16 storage throughput...")
+        s2 = PFSolver(wl.pf_args, wl.polycrystal, dtype="bfloat16", clean=False)
+        w2, e2, _ = time_steps(s2, s2.ini_cond(), wl.laser, ts, args.warmup, args.steps)
+        k2 = e2 / 1e3 / args.steps
+        out["bf16"] = {"value": upd * args.steps / w2, "ms_per_step": w2 / args.steps * 1e3,
+                       "roofline_frac": BYTES_PER_UPDATE["bfloat16"] * upd / k2 / 1e9 / HBM_PEAK_GBS,
+                       "tile": s2.tile, "zchunk": s2.zchunk}
+        log("FP64 mode throughput...")
+        s3 = PFSolver(wl.pf_args, wl.polycrystal, dtype="float64", clean=False)
+        n3 = max(10, args.steps // 10)
+        w3, e3, _ = time_steps(s3, s3.ini_cond(), wl.laser, ts, 3, n3)
+        out["f64"] = {"value": upd * n3 / w3, "ms_per_step": w3 / n3 * 1e3,
+                      "roofline_frac": BYTES_PER_UPDATE["float64"] * upd / (e3 / 1e3 / n3) / 1e9 / HBM_PEAK_GBS}
+        log(f"accuracy after {wl.steps} steps vs the FP64 mode...")
+        out["accuracy_vs_f64"] = accuracy(wl, wl.steps)
+    log(f"CPU baseline (~{args.cpu_seconds:.0f} s)...")
+    cb = cpu_baseline(wl, args.cpu_seconds)
+    out["cpu_baseline"] = cb
+    if not args.quick and not args.only_step:
+        try:
+            from pfkit import batchbench
+            log("batched (c3) and serving (c4)...")
+            out.update(batchbench.run_all(log))
+        except ImportError:
+            pass
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

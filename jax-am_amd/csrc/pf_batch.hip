@@ -1,0 +1,334 @@
+// Batched engine of libpfhip.so: many independent phase-field instances advanced
+// together.  Design:
+//   - Slot table (StepDev descriptors + BatchSlot step counters) lives in device
+//     memory; the device advances each slot's step index and ping-pong parity, so a
+//     captured graph stays valid while instances come and go.
+//   - Instances are grouped into classes (dtype, K, anisotropy, tile shape).  Each
+//     class has a device-resident work list of (slot, tile) items walked by one
+//     persistent launch per step with a fixed grid.
+//   - `chunk` steps (class launches + advance) are captured into one HIP graph and
+//     replayed; admitting or removing an instance only rewrites table entries and
+//     work lists (stream-ordered copies), and re-captures only when a new class
+//     appears.
+// Every tile of an instance runs exactly the per-cell arithmetic of pf_step with the
+// same tile shape, so an instance's trajectory does not depend on what else is in
+// the batch.
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "pf_host.h"
+
+namespace pf {
+
+__global__ __launch_bounds__(256) void advance_kernel(BatchSlot* __restrict__ slots, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  BatchSlot b = slots[s];
+  if (b.active && b.steps_left > 0) {
+    b.steps_left -= 1;
+    b.parity ^= 1;
+    b.n += 1;
+    slots[s] = b;
+  }
+}
+
+struct BatchClass {
+  int dtype, k, aniso, tile;
+  BatchLaunch fn;
+  int grid;
+  int off;    // offset of this class's work list in d_work
+  int count;  // host copy of the work-item count
+};
+
+struct SlotHost {
+  bool used = false;
+  int cls = -1;
+  int items = 0;  // work items of this slot
+};
+
+}  // namespace pf
+
+using namespace pf;
+
+struct pf_batch {
+  int max_inst = 0, max_tiles = 0;
+  static constexpr int kMaxClasses = 64;
+  hipStream_t stream = nullptr;
+  StepDev* d_desc = nullptr;
+  BatchSlot* d_slots = nullptr;
+  WorkItem* d_work = nullptr;
+  int* d_nwork = nullptr;
+  std::vector<SlotHost> slots;
+  std::vector<BatchClass> classes;
+  bool work_dirty = false;
+  hipGraphExec_t graph = nullptr;
+  hipGraph_t graph_def = nullptr;
+  int graph_chunk = 0;
+  int graph_classes = -1;
+  std::mutex mu;
+};
+
+namespace {
+
+int rebuild_work(pf_batch* b) {
+  // pack each class's work items contiguously, in slot order
+  std::vector<std::vector<WorkItem>> per(b->classes.size());
+  for (int s = 0; s < b->max_inst; ++s) {
+    const SlotHost& sh = b->slots[s];
+    if (!sh.used) continue;
+    for (int t = 0; t < sh.items; ++t) per[sh.cls].push_back(WorkItem{s, t});
+  }
+  std::vector<WorkItem> all;
+  std::vector<int> counts(pf_batch::kMaxClasses, 0);
+  for (size_t c = 0; c < b->classes.size(); ++c) {
+    b->classes[c].off = (int)all.size();
+    b->classes[c].count = (int)per[c].size();
+    counts[c] = (int)per[c].size();
+    all.insert(all.end(), per[c].begin(), per[c].end());
+  }
+  if ((int)all.size() > b->max_tiles) return fail(PF_E_ARG, "batch work list exceeds max_tiles");
+  hipError_t e = hipSuccess;
+  if (!all.empty())
+    e = hipMemcpyAsync(b->d_work, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_nwork, counts.data(), counts.size() * sizeof(int), hipMemcpyHostToDevice, b->stream);
+  // the host vectors above are pageable and local: finish the copies before they go away
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return hip_fail(e, "pf_batch work-list upload");
+  b->work_dirty = false;
+  return PF_OK;
+}
+
+int launch_step_all(pf_batch* b) {
+  for (size_t c = 0; c < b->classes.size(); ++c) {
+    const BatchClass& k = b->classes[c];
+    hipError_t e = k.fn(b->d_desc, b->d_slots, b->d_work + k.off, b->d_nwork + c, k.grid, b->stream);
+    if (e != hipSuccess) return hip_fail(e, "pf_batch step launch");
+  }
+  advance_kernel<<<(b->max_inst + 255) / 256, 256, 0, b->stream>>>(b->d_slots, b->max_inst);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? PF_OK : hip_fail(e, "pf_batch advance launch");
+}
+
+void drop_graph(pf_batch* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->graph_def) (void)hipGraphDestroy(b->graph_def);
+  b->graph = nullptr;
+  b->graph_def = nullptr;
+  b->graph_chunk = 0;
+  b->graph_classes = -1;
+}
+
+int ensure_graph(pf_batch* b, int chunk) {
+  if (b->graph && b->graph_chunk == chunk && b->graph_classes == (int)b->classes.size()) return PF_OK;
+  drop_graph(b);
+  hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+  int rc = PF_OK;
+  for (int i = 0; i < chunk && rc == PF_OK; ++i) rc = launch_step_all(b);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  hipGraphExec_t ex = nullptr;
+  e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  b->graph_def = g;
+  b->graph = ex;
+  b->graph_chunk = chunk;
+  b->graph_classes = (int)b->classes.size();
+  return PF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pf_batch_t* pf_batch_create(int32_t max_instances, int32_t max_tiles, void* stream) {
+  if (max_instances < 1 || max_tiles < 1) {
+    set_error("max_instances and max_tiles must be >= 1");
+    return nullptr;
+  }
+  pf_batch* b = new pf_batch();
+  b->max_inst = max_instances;
+  b->max_tiles = max_tiles;
+  b->stream = (hipStream_t)stream;
+  b->slots.resize(max_instances);
+  hipError_t e = hipMalloc((void**)&b->d_desc, sizeof(StepDev) * max_instances);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_slots, sizeof(BatchSlot) * max_instances);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_work, sizeof(WorkItem) * max_tiles);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_nwork, sizeof(int) * pf_batch::kMaxClasses);
+  if (e == hipSuccess) {
+    std::vector<BatchSlot> z(max_instances);
+    memset(z.data(), 0, z.size() * sizeof(BatchSlot));
+    std::vector<int> zc(pf_batch::kMaxClasses, 0);
+    e = hipMemcpyAsync(b->d_slots, z.data(), z.size() * sizeof(BatchSlot), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(b->d_nwork, zc.data(), zc.size() * sizeof(int), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  }
+  if (e != hipSuccess) {
+    hip_fail(e, "pf_batch_create");
+    pf_batch_destroy(b);
+    return nullptr;
+  }
+  return b;
+}
+
+void pf_batch_destroy(pf_batch_t* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_slots) (void)hipFree(b->d_slots);
+  if (b->d_work) (void)hipFree(b->d_work);
+  if (b->d_nwork) (void)hipFree(b->d_nwork);
+  delete b;
+}
+
+int pf_batch_add(pf_batch_t* b, const pf_batch_inst_t* inst) {
+  if (!b || !inst) return fail(PF_E_ARG, "NULL batch or instance");
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (!inst->eta_dev[0] || !inst->eta_dev[1] || inst->eta_dev[0] == inst->eta_dev[1])
+    return fail(PF_E_ARG, "an instance needs two distinct state buffers");
+  if (((uintptr_t)inst->eta_dev[0] | (uintptr_t)inst->eta_dev[1]) & 15)
+    return fail(PF_E_ARG, "eta buffers must be 16-byte aligned");
+  if (inst->steps < 0) return fail(PF_E_ARG, "steps must be >= 0");
+  if (inst->zchunk < 0) return fail(PF_E_ARG, "zchunk must be >= 0");
+  int slot = -1;
+  for (int s = 0; s < b->max_inst; ++s)
+    if (!b->slots[s].used) {
+      slot = s;
+      break;
+    }
+  if (slot < 0) return fail(PF_E_STATE, "no free batch slot");
+  StepDev a;
+  int rc = make_stepdev(&inst->p, inst->dtype, inst->T_dev, inst->axes_dev, inst->zchunk, a);
+  if (rc) return rc;
+  const bool aniso = inst->p.anisotropy > 0.0;
+  const StepEntry* e = find_entry(inst->dtype, inst->p.num_oris, aniso, inst->tile);
+  int items, tile = -1, grid;
+  BatchLaunch fn;
+  if (e) {
+    set_tiles(a, e->tx, e->ty, inst->zchunk);
+    items = a.tiles_x * a.tiles_y * a.tiles_z;
+    tile = e->tile;
+    fn = e->batch;
+    grid = 1024;
+  } else {
+    if (inst->tile > 0 && entry_count(inst->dtype, inst->p.num_oris, aniso) > 0)
+      return fail(PF_E_ARG, "no such tile shape");
+    const long long n = (long long)inst->p.nz * inst->p.ny * inst->p.nx;
+    if ((n + 255) / 256 >= (1LL << 31)) return fail(PF_E_ARG, "instance too large");
+    items = (int)((n + 255) / 256);
+    fn = generic_batch(inst->dtype, aniso);
+    grid = 2048;
+  }
+  int total = 0;
+  for (const SlotHost& sh : b->slots)
+    if (sh.used) total += sh.items;
+  if (total + items > b->max_tiles) return fail(PF_E_STATE, "batch work list full (max_tiles)");
+  int cls = -1;
+  for (size_t c = 0; c < b->classes.size(); ++c) {
+    const BatchClass& k = b->classes[c];
+    if (k.dtype == inst->dtype && k.k == inst->p.num_oris && k.aniso == (int)aniso && k.tile == tile) cls = (int)c;
+  }
+  if (cls < 0) {
+    if ((int)b->classes.size() >= pf_batch::kMaxClasses) return fail(PF_E_STATE, "too many instance classes");
+    b->classes.push_back(BatchClass{inst->dtype, inst->p.num_oris, (int)aniso, tile, fn, grid, 0, 0});
+    cls = (int)b->classes.size() - 1;
+  }
+  BatchSlot st;
+  memset(&st, 0, sizeof(st));
+  st.steps_left = inst->steps;
+  st.parity = 0;
+  st.n = 0;
+  st.active = 1;
+  st.t0 = inst->t0;
+  st.dt = inst->dt;
+  st.eta[0] = inst->eta_dev[0];
+  st.eta[1] = inst->eta_dev[1];
+  hipError_t he = hipMemcpyAsync(b->d_desc + slot, &a, sizeof(StepDev), hipMemcpyHostToDevice, b->stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(b->d_slots + slot, &st, sizeof(BatchSlot), hipMemcpyHostToDevice, b->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(b->stream);  // a and st are locals
+  if (he != hipSuccess) return hip_fail(he, "pf_batch_add upload");
+  b->slots[slot].used = true;
+  b->slots[slot].cls = cls;
+  b->slots[slot].items = items;
+  b->work_dirty = true;
+  return slot;
+}
+
+int pf_batch_remove(pf_batch_t* b, int32_t slot) {
+  if (!b) return fail(PF_E_ARG, "NULL batch");
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (slot < 0 || slot >= b->max_inst || !b->slots[slot].used) return fail(PF_E_ARG, "slot not in use");
+  BatchSlot st;
+  memset(&st, 0, sizeof(st));
+  hipError_t he = hipMemcpyAsync(b->d_slots + slot, &st, sizeof(BatchSlot), hipMemcpyHostToDevice, b->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(b->stream);  // st is a local
+  if (he != hipSuccess) return hip_fail(he, "pf_batch_remove");
+  b->slots[slot] = SlotHost();
+  b->work_dirty = true;
+  return PF_OK;
+}
+
+int pf_batch_run(pf_batch_t* b, int32_t steps, int32_t chunk) {
+  if (!b) return fail(PF_E_ARG, "NULL batch");
+  if (steps < 0) return fail(PF_E_ARG, "steps must be >= 0");
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (b->work_dirty) {
+    int rc = rebuild_work(b);
+    if (rc) return rc;
+  }
+  int done = 0;
+  if (chunk > 0 && steps >= chunk) {
+    int rc = ensure_graph(b, chunk);
+    if (rc) return rc;
+    for (; done + chunk <= steps; done += chunk) {
+      hipError_t e = hipGraphLaunch(b->graph, b->stream);
+      if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    }
+  }
+  for (; done < steps; ++done) {
+    int rc = launch_step_all(b);
+    if (rc) return rc;
+  }
+  return PF_OK;
+}
+
+int pf_batch_status(pf_batch_t* b, int32_t* steps_left, double* t, int32_t* parity, int32_t n) {
+  if (!b) return fail(PF_E_ARG, "NULL batch");
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (n > b->max_inst) n = b->max_inst;
+  std::vector<BatchSlot> h(b->max_inst);
+  hipError_t e = hipMemcpyAsync(h.data(), b->d_slots, sizeof(BatchSlot) * b->max_inst, hipMemcpyDeviceToHost,
+                                b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return hip_fail(e, "pf_batch_status");
+  for (int s = 0; s < n; ++s) {
+    if (steps_left) steps_left[s] = b->slots[s].used ? h[s].steps_left : -1;
+    if (t) t[s] = h[s].t0 + (double)h[s].n * h[s].dt;
+    if (parity) parity[s] = h[s].parity;
+  }
+  return PF_OK;
+}
+
+int pf_batch_active(const pf_batch_t* b) {
+  if (!b) return 0;
+  int c = 0;
+  for (const SlotHost& s : b->slots) c += s.used ? 1 : 0;
+  return c;
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+// Kernel entry points of the fused step (single instance and batched), their
+// launchers, and the dispatch-table entry type.  Instantiated per storage dtype in
+// pf_inst_*.hip.
+#pragma once
+
+#include "pf_device.h"
+
+namespace pf {
+
+// Per-slot device state of the batched engine.  The step index n gives
+// t_prev = t0 + n*dt and t_crt = t0 + (n+1)*dt (the reference's np.arange time grid,
+// applications/phase_field/simple/example.py:52), so h = t_crt - t_prev per step.
+struct BatchSlot {
+  int steps_left;
+  int parity;  // state lives in eta[parity]
+  int n;
+  int active;
+  double t0, dt;
+  void* eta[2];
+};
+
+struct WorkItem {
+  int slot;
+  int tile;
+};
+
+template <int K, class TS, bool ANISO, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void step_kernel(const StepDev a) {
+  using TC = typename Traits<TS>::TC;
+  __shared__ TC smem[2 * Tile<TS, K, TX, TY>::PLANE];
+  const int n = a.tiles_x * a.tiles_y * a.tiles_z;
+  const int t = xcd_swizzle(blockIdx.x, n);
+  const int tx = t % a.tiles_x;
+  const int ty = (t / a.tiles_x) % a.tiles_y;
+  const int tz = t / (a.tiles_x * a.tiles_y);
+  march_tile<K, TS, ANISO, TX, TY>(a, tx, ty, tz, smem);
+}
+
+template <class TS, bool ANISO, bool RHS = false>
+__global__ __launch_bounds__(256) void generic_kernel(const StepDev a) {
+  const long long n = (long long)a.nz * a.ny * a.nx;
+  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell < n) generic_cell<TS, ANISO, RHS>(a, cell);
+}
+
+// Instance view for one slot at its current step.
+__device__ __forceinline__ StepDev slot_view(const StepDev& d, const BatchSlot& s) {
+  StepDev a = d;
+  a.in = s.eta[s.parity];
+  a.out = s.eta[s.parity ^ 1];
+  const double tp = s.t0 + (double)s.n * s.dt;
+  const double tc = s.t0 + (double)(s.n + 1) * s.dt;
+  a.t_prev = tp;
+  a.hstep = tc - tp;
+  return a;
+}
+
+// Persistent batched step: a fixed grid walks the device-resident work list (slot,
+// tile) of one (dtype, K, anisotropy, tile shape) class.  Every workgroup exits after
+// the list is exhausted; slots with no steps left are skipped.
+template <int K, class TS, bool ANISO, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void batch_kernel(const StepDev* __restrict__ descs,
+                                                      const BatchSlot* __restrict__ slots,
+                                                      const WorkItem* __restrict__ work,
+                                                      const int* __restrict__ nwork) {
+  using TC = typename Traits<TS>::TC;
+  __shared__ TC smem[2 * Tile<TS, K, TX, TY>::PLANE];
+  const int nw = *nwork;
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const WorkItem it = work[w];
+    const BatchSlot s = slots[it.slot];
+    if (s.steps_left <= 0 || !s.active) continue;
+    const StepDev a = slot_view(descs[it.slot], s);
+    const int tx = it.tile % a.tiles_x;
+    const int ty = (it.tile / a.tiles_x) % a.tiles_y;
+    const int tz = it.tile / (a.tiles_x * a.tiles_y);
+    march_tile<K, TS, ANISO, TX, TY>(a, tx, ty, tz, smem);
+    __syncthreads();  // LDS is reused by the next tile
+  }
+}
+
+// Batched generic-K step: work items are (slot, block of 256 cells).
+template <class TS, bool ANISO>
+__global__ __launch_bounds__(256) void batch_generic_kernel(const StepDev* __restrict__ descs,
+                                                           const BatchSlot* __restrict__ slots,
+                                                           const WorkItem* __restrict__ work,
+                                                           const int* __restrict__ nwork) {
+  const int nw = *nwork;
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const WorkItem it = work[w];
+    const BatchSlot s = slots[it.slot];
+    if (s.steps_left <= 0 || !s.active) continue;
+    const StepDev a = slot_view(descs[it.slot], s);
+    const long long n = (long long)a.nz * a.ny * a.nx;
+    const long long cell = (long long)it.tile * 256 + threadIdx.x;
+    if (cell < n) generic_cell<TS, ANISO>(a, cell);
+  }
+}
+
+using StepLaunch = hipError_t (*)(const StepDev&, hipStream_t);
+using BatchLaunch = hipError_t (*)(const StepDev*, const BatchSlot*, const WorkItem*, const int*, int grid,
+                                   hipStream_t);
+
+struct StepEntry {
+  int dtype, k, aniso, tile, tx, ty;
+  StepLaunch step;
+  BatchLaunch batch;
+  size_t lds;
+};
+
+template <int K, class TS, bool ANISO, int TX, int TY>
+hipError_t launch_step(const StepDev& a, hipStream_t s) {
+  const int n = a.tiles_x * a.tiles_y * a.tiles_z;
+  step_kernel<K, TS, ANISO, TX, TY><<<dim3(n), dim3(TX * TY), 0, s>>>(a);
+  return hipGetLastError();
+}
+
+template <int K, class TS, bool ANISO, int TX, int TY>
+hipError_t launch_batch(const StepDev* d, const BatchSlot* sl, const WorkItem* w, const int* nw, int grid,
+                        hipStream_t s) {
+  batch_kernel<K, TS, ANISO, TX, TY><<<dim3(grid), dim3(TX * TY), 0, s>>>(d, sl, w, nw);
+  return hipGetLastError();
+}
+
+template <class TS, bool ANISO, bool RHS = false>
+hipError_t launch_generic(const StepDev& a, hipStream_t s) {
+  const long long n = (long long)a.nz * a.ny * a.nx;
+  generic_kernel<TS, ANISO, RHS><<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(a);
+  return hipGetLastError();
+}
+
+template <class TS, bool ANISO>
+hipError_t launch_batch_generic(const StepDev* d, const BatchSlot* sl, const WorkItem* w, const int* nw, int grid,
+                                hipStream_t s) {
+  batch_generic_kernel<TS, ANISO><<<dim3(grid), dim3(256), 0, s>>>(d, sl, w, nw);
+  return hipGetLastError();
+}
+
+#define PF_ENTRY(DT, TS, K, A, T, TX, TY)                                                             \
+  {DT, K, A, T, TX, TY, &launch_step<K, TS, (A != 0), TX, TY>, &launch_batch<K, TS, (A != 0), TX, TY>, \
+   Tile<TS, K, TX, TY>::LDS_BYTES}
+
+// instantiation tables, one per compiled part of pf_inst.hip (see Makefile):
+// parts 0-3 F32, 4-7 BF16, 8-11 F64, each over a K range
+constexpr int kInstParts = 12;
+const StepEntry* step_table_part_0(int* n);
+const StepEntry* step_table_part_1(int* n);
+const StepEntry* step_table_part_2(int* n);
+const StepEntry* step_table_part_3(int* n);
+const StepEntry* step_table_part_4(int* n);
+const StepEntry* step_table_part_5(int* n);
+const StepEntry* step_table_part_6(int* n);
+const StepEntry* step_table_part_7(int* n);
+const StepEntry* step_table_part_8(int* n);
+const StepEntry* step_table_part_9(int* n);
+const StepEntry* step_table_part_10(int* n);
+const StepEntry* step_table_part_11(int* n);
+// generic-K launchers by dtype
+StepLaunch generic_step(int dtype, bool aniso);
+BatchLaunch generic_batch(int dtype, bool aniso);
+
+}  // namespace pf

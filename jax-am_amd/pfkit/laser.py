@@ -1,0 +1,98 @@
+"""Temperature descriptors evaluated inside the fused step kernel.
+
+In the reference the app computes T on its own (one extra jitted launch and an
+(N, 1) array per step) and passes it to ``stepper`` through ``ode_params``.  Passing
+one of these descriptors instead lets the kernel evaluate T at each cell centroid
+(FP64) with no temperature array in memory.  The reference's array form is still
+accepted by ``PFSolver.stepper`` (PF_T_BUFFER).
+"""
+from dataclasses import dataclass
+from typing import Tuple
+
+from . import _lib
+
+
+@dataclass
+class Rosenthal:
+    """Moving point source, applications/phase_field/simple/example.py:21-37
+    (general direction: multi_scan/example.py:33-49).
+
+    T = T_ambient + Q/(2 pi kappa_T) / R * exp(-vel/(2 alpha) (R + (X,Y,Z).dir)),
+    source at pos0 + vel t dir.  T_cap > 0 caps T (multi_scan/example.py:47).
+    """
+    pos0: Tuple[float, float, float]
+    vel: float
+    Q: float = 25.0
+    alpha: float = 5.2e-6
+    kappa_T: float = 27.0
+    T_ambient: float = 300.0
+    dir: Tuple[float, float, float] = (1.0, 0.0, 0.0)
+    T_cap: float = 0.0
+
+    @classmethod
+    def simple_app(cls, pf_args):
+        """The parameters of simple/example.py:26-33: x0 = 0.2 Lx, y = Ly/2, z = Lz,
+        vel = 0.6 Lx / t_OFF."""
+        return cls(pos0=(0.2 * pf_args["domain_x"], 0.5 * pf_args["domain_y"], pf_args["domain_z"]),
+                   vel=0.6 * pf_args["domain_x"] / pf_args["t_OFF"], T_ambient=pf_args["T_ambient"])
+
+    def fill(self, L):
+        L.model = _lib.PF_T_ROSENTHAL
+        L.T_ambient = self.T_ambient
+        for d in range(3):
+            L.pos0[d] = self.pos0[d]
+            L.dir[d] = self.dir[d]
+        L.vel = self.vel
+        L.Q, L.kappa_T, L.alpha, L.T_cap = self.Q, self.kappa_T, self.alpha, self.T_cap
+
+    def as_dict(self):
+        return dict(model="rosenthal", pos0=self.pos0, vel=self.vel, Q=self.Q, alpha=self.alpha,
+                    kappa_T=self.kappa_T, T_ambient=self.T_ambient, dir=self.dir, T_cap=self.T_cap)
+
+
+@dataclass
+class Gaussian:
+    """Moving Gaussian spot: T = T_ambient + dT_peak exp(-(X^2+Y^2)/(2 sigma_xy^2) - Z^2/(2 sigma_z^2)),
+    centre at pos0 + vel t dir."""
+    pos0: Tuple[float, float, float]
+    vel: float
+    dT_peak: float
+    sigma_xy: float
+    sigma_z: float
+    T_ambient: float = 300.0
+    dir: Tuple[float, float, float] = (1.0, 0.0, 0.0)
+
+    def fill(self, L):
+        L.model = _lib.PF_T_GAUSSIAN
+        L.T_ambient = self.T_ambient
+        for d in range(3):
+            L.pos0[d] = self.pos0[d]
+            L.dir[d] = self.dir[d]
+        L.vel = self.vel
+        L.dT_peak, L.sigma_xy, L.sigma_z = self.dT_peak, self.sigma_xy, self.sigma_z
+
+    def as_dict(self):
+        return dict(model="gaussian", pos0=self.pos0, vel=self.vel, dT_peak=self.dT_peak,
+                    sigma_xy=self.sigma_xy, sigma_z=self.sigma_z, T_ambient=self.T_ambient, dir=self.dir)
+
+
+@dataclass
+class Quench:
+    """Directional solidification, solidification/example.py:21-34: T = T_ref + grad_T z - cool_rate t."""
+    T_ref: float
+    grad_T: float
+    cool_rate: float
+
+    @classmethod
+    def solidification_app(cls, pf_args, vel=0.2, thermal_grad=5.e5):
+        return cls(T_ref=pf_args["T_liquidus"], grad_T=thermal_grad, cool_rate=thermal_grad * vel)
+
+    def fill(self, L):
+        L.model = _lib.PF_T_QUENCH
+        L.T_ref, L.grad_T, L.cool_rate = self.T_ref, self.grad_T, self.cool_rate
+
+    def as_dict(self):
+        return dict(model="quench", T_ref=self.T_ref, grad_T=self.grad_T, cool_rate=self.cool_rate)
+
+
+LASERS = (Rosenthal, Gaussian, Quench)

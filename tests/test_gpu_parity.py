@@ -1,0 +1,228 @@
+"""GPU parity of the fused HIP step against the FP64 oracle (oracle/pf_oracle.py).
+
+Tolerances (written here, checked per test):
+  F64 storage/compute: relative L-inf <= 1e-12 per step (max|d| / max|eta|).
+  F32 storage/compute: |d| <= 2e-6 * max|eta| + 1e-7 per step.
+  BF16 storage, F32 compute: the only extra error is the final round-to-nearest-even to
+  BF16, so |d| <= 2^-8 |eta_new| + 2e-6 (one bf16 half-ulp plus FP32 compute error).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import pf_oracle as O
+from pfkit import _lib
+from pfkit.allen_cahn import PFSolver
+from pfkit.config import make_params
+from problems import args, gaussian, poly, quench, random_eta, rosenthal
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(a, pc, dtype, cuda, **kw):
+    return PFSolver(a, pc, dtype=dtype, device=cuda, clean=False, tile=kw.pop("tile", -1),
+                    zchunk=kw.pop("zchunk", 0), **kw)
+
+
+def _oracle_step(eta, a, pc, L, t0, t1, T=None):
+    cent = O.centroids(a["Nz"], a["Ny"], a["Nx"], pc.mesh_h_xyz)
+    if T is None:
+        T = O.laser_T(L.as_dict(), cent, t0)
+    return O.explicit_euler(eta, t0, t1, T, a, pc.mesh_h_xyz, pc.unique_grain_directions)
+
+
+def _gpu_step(s, eta_np, L, t0, t1, dtype, tile=-1, zchunk=0):
+    eta = torch.as_tensor(eta_np, dtype=dtype, device=s.device)
+    out = torch.empty_like(eta)
+    s.state_rhs.step(eta, out, L, t0, t1, tile, zchunk)
+    torch.cuda.synchronize()
+    return out
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+CASES = [
+    # (Nx, Ny, Nz, K, aniso, laser, force_liquid)
+    (16, 12, 10, 10, 0.0, "gauss", False),
+    (16, 12, 10, 10, 0.15, "gauss", True),
+    (33, 17, 9, 20, 0.15, "rosen", False),
+    (40, 9, 13, 3, 0.3, "quench", True),
+    (7, 5, 6, 5, 0.15, "rosen", True),
+    (24, 20, 12, 16, 0.0, "gauss", True),
+    (12, 10, 8, 25, 0.15, "gauss", False),   # generic-K kernel
+    (9, 7, 5, 33, 0.0, "quench", False),     # generic-K kernel
+    (1, 1, 12, 4, 0.15, "gauss", False),     # degenerate 1 x 1 x Nz column
+    (70, 1, 1, 6, 0.15, "gauss", False),     # degenerate line
+]
+
+
+def _laser(kind, Nx, Ny, Nz):
+    return {"gauss": gaussian(Nx, Ny, Nz), "rosen": rosenthal(Nx, Ny, Nz), "quench": quench(Nz)}[kind]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}x{c[1]}x{c[2]}k{c[3]}e{c[4]}{c[5]}fl{int(c[6])}" for c in CASES])
+def test_f64_step_matches_oracle(case, cuda):
+    Nx, Ny, Nz, K, eps, lk, fl = case
+    print(f"case {case}", flush=True)
+    a = args(Nx, Ny, Nz, K, anisotropy=eps, force_liquid=fl)
+    pc = poly(Nx, Ny, Nz, K)
+    L = _laser(lk, Nx, Ny, Nz)
+    eta = random_eta(Nx * Ny * Nz, K, seed=Nx + K)
+    t0, t1 = 0.3, 0.3 + 0.02
+    s = _solver(a, pc, torch.float64, cuda)
+    got = _gpu_step(s, eta, L, t0, t1, torch.float64).cpu().numpy()
+    ref = _oracle_step(eta, a, pc, L, t0, t1)
+    assert _rel(got, ref) <= 1e-12, _rel(got, ref)
+
+
+@pytest.mark.parametrize("case", CASES[:6], ids=[f"{c[0]}x{c[1]}x{c[2]}k{c[3]}" for c in CASES[:6]])
+def test_f32_step_matches_oracle(case, cuda):
+    Nx, Ny, Nz, K, eps, lk, fl = case
+    print(f"case {case}", flush=True)
+    a = args(Nx, Ny, Nz, K, anisotropy=eps, force_liquid=fl)
+    pc = poly(Nx, Ny, Nz, K)
+    L = _laser(lk, Nx, Ny, Nz)
+    eta = random_eta(Nx * Ny * Nz, K, seed=Nx + K).astype(np.float32).astype(np.float64)
+    s = _solver(a, pc, torch.float32, cuda)
+    got = _gpu_step(s, eta, L, 0.3, 0.32, torch.float32).cpu().numpy().astype(np.float64)
+    ref = _oracle_step(eta, a, pc, L, 0.3, 0.32)
+    assert np.max(np.abs(got - ref)) <= 2e-6 * np.max(np.abs(ref)) + 1e-7
+
+
+@pytest.mark.parametrize("case", CASES[:4], ids=[f"{c[0]}x{c[1]}x{c[2]}k{c[3]}" for c in CASES[:4]])
+def test_bf16_step_matches_oracle(case, cuda):
+    Nx, Ny, Nz, K, eps, lk, fl = case
+    a = args(Nx, Ny, Nz, K, anisotropy=eps, force_liquid=fl)
+    pc = poly(Nx, Ny, Nz, K)
+    L = _laser(lk, Nx, Ny, Nz)
+    eta_b = torch.as_tensor(random_eta(Nx * Ny * Nz, K, seed=Nx + K)).to(torch.bfloat16)
+    eta = eta_b.to(torch.float64).numpy()
+    s = _solver(a, pc, torch.bfloat16, cuda)
+    got = _gpu_step(s, eta, L, 0.3, 0.32, torch.bfloat16).to(torch.float64).cpu().numpy()
+    ref = _oracle_step(eta, a, pc, L, 0.3, 0.32)
+    assert np.all(np.abs(got - ref) <= 2.0 ** -8 * np.abs(ref) + 2e-6)
+
+
+def test_tile_shapes_and_generic_kernel_agree_bitwise(cuda):
+    Nx, Ny, Nz, K = 37, 21, 19, 10
+    a = args(Nx, Ny, Nz, K, anisotropy=0.15, force_liquid=True)
+    pc = poly(Nx, Ny, Nz, K)
+    L = gaussian(Nx, Ny, Nz)
+    eta = random_eta(Nx * Ny * Nz, K, seed=5)
+    for dtype in (torch.float64, torch.float32, torch.bfloat16):
+        s = _solver(a, pc, dtype, cuda)
+        lib = _lib.load()
+        nt = lib.pf_tile_count(_lib.TORCH_TO_PF[dtype], K)
+        assert nt >= 2
+        base = _gpu_step(s, eta, L, 0.1, 0.12, dtype, tile=-2)  # generic kernel
+        for tile in range(nt):
+            for zc in (0, 1, 3, 7, Nz):
+                got = _gpu_step(s, eta, L, 0.1, 0.12, dtype, tile=tile, zchunk=zc)
+                assert torch.equal(got, base), (dtype, tile, zc)
+
+
+def test_t_buffer_matches_laser_and_oracle(cuda):
+    Nx, Ny, Nz, K = 20, 14, 9, 10
+    a = args(Nx, Ny, Nz, K, anisotropy=0.15, force_liquid=True)
+    pc = poly(Nx, Ny, Nz, K)
+    L = rosenthal(Nx, Ny, Nz)
+    eta = random_eta(Nx * Ny * Nz, K, seed=9)
+    cent = O.centroids(Nz, Ny, Nx, pc.mesh_h_xyz)
+    T = O.laser_T(L.as_dict(), cent, 0.4)
+    s = _solver(a, pc, torch.float64, cuda)
+    for Tin in (T, T.astype(np.float32)):
+        (eta_t, t), _ = s.stepper((torch.as_tensor(eta, device=cuda), 0.4), 0.43, [torch.as_tensor(Tin, device=cuda)])
+        ref = O.explicit_euler(eta, 0.4, 0.43, Tin.astype(np.float64), a, pc.mesh_h_xyz, pc.unique_grain_directions)
+        assert t == 0.43
+        assert _rel(eta_t.cpu().numpy(), ref) <= 1e-12
+
+
+def test_multistep_f64_trajectory(cuda):
+    Nx, Ny, Nz, K = 24, 16, 10, 10
+    a = args(Nx, Ny, Nz, K, anisotropy=0.15, force_liquid=True)
+    pc = poly(Nx, Ny, Nz, K)
+    L = gaussian(Nx, Ny, Nz, vel=3.0)
+    eta0 = O.ini_cond(pc.cell_ori_inds, K)
+    ts = np.arange(0.0, 30 * 0.03 + 1e-10, 0.03)
+    s = _solver(a, pc, torch.float64, cuda)
+    state = (s.ini_cond(), ts[0])
+    for t in ts[1:]:
+        state, _ = s.stepper(state, t, [L])
+    ref = O.run(eta0, ts, a, pc.mesh_h_xyz, pc.unique_grain_directions, laser=L.as_dict(),
+                cent=O.centroids(Nz, Ny, Nx, pc.mesh_h_xyz))
+    assert _rel(state[0].cpu().numpy(), ref) <= 1e-11
+
+
+def test_rhs_matches_oracle(cuda):
+    Nx, Ny, Nz, K = 13, 11, 7, 10
+    for eps in (0.0, 0.15):
+        a = args(Nx, Ny, Nz, K, anisotropy=eps)
+        pc = poly(Nx, Ny, Nz, K)
+        L = gaussian(Nx, Ny, Nz)
+        eta = random_eta(Nx * Ny * Nz, K, seed=2)
+        s = _solver(a, pc, torch.float64, cuda)
+        got = s.state_rhs(torch.as_tensor(eta, device=cuda), 0.2, [L]).cpu().numpy()
+        T = O.laser_T(L.as_dict(), O.centroids(Nz, Ny, Nx, pc.mesh_h_xyz), 0.2)
+        ref = O.state_rhs(eta, T, a, pc.mesh_h_xyz, pc.unique_grain_directions)
+        assert _rel(got, ref) <= 1e-12
+
+
+def test_temperature_models(cuda):
+    Nx, Ny, Nz, K = 16, 12, 10, 4
+    a = args(Nx, Ny, Nz, K)
+    pc = poly(Nx, Ny, Nz, K)
+    s = _solver(a, pc, torch.float64, cuda)
+    cent = O.centroids(Nz, Ny, Nx, pc.mesh_h_xyz)
+    for L in (rosenthal(Nx, Ny, Nz), gaussian(Nx, Ny, Nz), quench(Nz)):
+        for t in (0.0, 1.7):
+            got = s.temperature(L, t)
+            ref = O.laser_T(L.as_dict(), cent, t)
+            assert np.max(np.abs(got - ref) / np.abs(ref)) <= 1e-14
+
+
+def test_init_argmax_and_reduce(cuda):
+    Nx, Ny, Nz, K = 19, 13, 11, 10
+    a = args(Nx, Ny, Nz, K, anisotropy=0.15)
+    pc = poly(Nx, Ny, Nz, K)
+    L = gaussian(Nx, Ny, Nz)
+    s = _solver(a, pc, torch.float64, cuda)
+    eta0 = s.ini_cond()
+    assert np.array_equal(eta0.cpu().numpy(), O.ini_cond(pc.cell_ori_inds, K))
+    eta = random_eta(Nx * Ny * Nz, K, seed=4)
+    eta[3, :] = 0.5          # ties: first maximum wins
+    eta[5, 2] = np.nan       # NaN counts as the maximum
+    et = torch.as_tensor(eta, device=cuda)
+    am = s.argmax(et).cpu().numpy()
+    assert np.array_equal(am, np.argmax(eta, axis=1))
+    eta[5, 2] = 0.25
+    et = torch.as_tensor(eta, device=cuda)
+    d = s.diagnostics(et, L, t=0.5, eta0=eta0)
+    T = O.laser_T(L.as_dict(), O.centroids(Nz, Ny, Nx, pc.mesh_h_xyz), 0.5)
+    Fl, Fg = O.free_energy(eta, T, a, pc.mesh_h_xyz)
+    assert abs(d["F_local"] - Fl) <= 1e-12 * abs(Fl)
+    assert abs(d["F_grad"] - Fg) <= 1e-12 * abs(Fg)
+    assert np.allclose(d["volumes"], O.volumes(eta, pc.mesh_h_xyz), rtol=1e-13, atol=0)
+    ins = O.inspect(eta, O.ini_cond(pc.cell_ori_inds, K), T, a)
+    assert d["changed"] == round(ins["change_pct"] / 100 * Nx * Ny * Nz)
+    assert d["liquid"] == round(ins["liquid_pct"] / 100 * Nx * Ny * Nz)
+    assert d["max_T"] == pytest.approx(ins["max_T"], rel=1e-14)
+    assert np.array_equal(d["argmax_counts"], np.bincount(np.argmax(eta, axis=1), minlength=K))
+    assert d["nonfinite"] == 0
+    # inspect_sol raises on non-finite state, like allen_cahn.py:179-180
+    eta[7, 1] = np.inf
+    with pytest.raises(ValueError):
+        s.inspect_sol(torch.as_tensor(eta, device=cuda), eta0, L, np.arange(3) * 0.1, 1)
+
+
+def test_reduce_is_deterministic_and_dtype_consistent(cuda):
+    Nx, Ny, Nz, K = 64, 48, 20, 20
+    a = args(Nx, Ny, Nz, K, anisotropy=0.15)
+    pc = poly(Nx, Ny, Nz, K)
+    L = gaussian(Nx, Ny, Nz)
+    s = _solver(a, pc, torch.float32, cuda)
+    eta = torch.as_tensor(random_eta(Nx * Ny * Nz, K, seed=8), dtype=torch.float32, device=cuda)
+    d1 = s.diagnostics(eta, L, 0.2)
+    d2 = s.diagnostics(eta, L, 0.2)
+    assert d1["F"] == d2["F"] and np.array_equal(d1["volumes"], d2["volumes"])
