@@ -138,7 +138,20 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tile", type=int, default=None)
     ap.add_argument("--zchunk", type=int, default=None)
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="N",
+                    help="only run N device copies of one FP32 eta array (335.5 MB read + written each); "
+                         "the profiler's byte counters on them calibrate the traffic figure")
     args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+        x = torch.ones(256 * 256 * 64 * 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(args.calib_copy):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy": args.calib_copy, "bytes_read_per_copy": x.numel() * 4,
+                          "bytes_written_per_copy": x.numel() * 4}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
@@ -193,9 +206,9 @@ def main():
                       "roofline_frac": BYTES_PER_UPDATE["float64"] * upd / (e3 / 1e3 / n3) / 1e9 / HBM_PEAK_GBS}
         log(f"accuracy after {wl.steps} steps vs the FP64 mode...")
         out["accuracy_vs_f64"] = accuracy(wl, wl.steps)
-    log(f"CPU baseline (~{args.cpu_seconds:.0f} s)...")
-    cb = cpu_baseline(wl, args.cpu_seconds)
-    out["cpu_baseline"] = cb
+    if not args.only_step:
+        log(f"CPU baseline (~{args.cpu_seconds:.0f} s)...")
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     if not args.quick and not args.only_step:
         try:
             from pfkit import batchbench

@@ -145,6 +145,16 @@ int pf_init_onehot(void* eta_dev, int32_t dtype, const int32_t* cell_ori_inds_de
                    int64_t n_cells, int32_t k, void* stream);
 
 /*
+ * Voronoi grain map on the grid: cell c (centroid ((i+1/2)h_x, (j+1/2)h_y, (k+1/2)h_z)) takes
+ * the nearest of n_seeds seeds (seeds_dev: F64 [n_seeds][3]; ties go to the lower index) and
+ * out_dev[c] = grain_ori_dev[seed] (or the seed index when grain_ori_dev is NULL).  Replaces the
+ * Neper tessellation + hex meshing of jax_am/phase_field/neper.py:5-22 and the grain -> orientation
+ * lookup of utils.py:39-40 for request-time initial states.  Uses p->nz/ny/nx and p->h only.
+ */
+int pf_voronoi(const double* seeds_dev, int32_t n_seeds, const int32_t* grain_ori_dev,
+               const pf_params_t* p, int32_t* out_dev, void* stream);
+
+/*
  * argmax over orientations (first maximum wins, NaN counts as maximum, like numpy):
  * out_dev[c] = argmax_i eta[c, i].  Replaces the argmax of inspect_sol / write_sols
  * (allen_cahn.py:171-172,189).

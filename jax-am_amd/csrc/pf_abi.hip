@@ -87,6 +87,8 @@ int make_stepdev(const pf_params_t* p, int dtype, const void* T_dev, const void*
   a.zeta_window = p->zeta_sharpness > 0.0 ? 40.0 * p->T_liquidus / p->zeta_sharpness : INFINITY;
   a.eps = aniso ? p->anisotropy : 0.0;
   a.liquid_T = p->T_liquidus + p->liquid_shift;
+  a.stiff = fabs(a.m_over_ad) * (4.0 + 4.0 * fabs(p->gamma)) +
+            4.0 * fabs(a.kappa_ad) * (1.0 + a.eps) * (a.ihx2 + a.ihy2 + a.ihz2);
   a.T_amb = L.T_ambient;
   a.p0x = L.pos0[0];
   a.p0y = L.pos0[1];
@@ -153,9 +155,10 @@ const StepEntry* find_entry(int dtype, int k, bool aniso, int tile) {
 }
 
 int default_zchunk(int nz, int ny, int nx, int tx, int ty) {
-  // about 512 workgroups (2 per CU at 2 waves/SIMD on 256 CUs), at least 4 planes per march
+  // about 2048 workgroups (four rounds of 2 per CU on 256 CUs: shorter marches balance the
+  // data-dependent work better), at least 4 planes per march
   const long long txy = (long long)((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
-  long long chunks = (512 + txy / 2) / txy;
+  long long chunks = (2048 + txy / 2) / txy;
   if (chunks < 1) chunks = 1;
   int zc = (int)((nz + chunks - 1) / chunks);
   if (zc < 4) zc = 4;
@@ -218,6 +221,38 @@ __global__ __launch_bounds__(256) void onehot_kernel(TS* __restrict__ eta, const
   const long long c = e / k;
   const int i = (int)(e - c * k);
   eta[e] = from_tc<TS>(ori[c] == i ? (TC)1 : (TC)0);
+}
+
+// nearest seed of each cell centroid (seeds staged through LDS in blocks of 256)
+__global__ __launch_bounds__(256) void voronoi_kernel(const double* __restrict__ seeds, int ns,
+                                                     const int32_t* __restrict__ gori, int nz, int ny, int nx,
+                                                     double hx, double hy, double hz, int32_t* __restrict__ out) {
+  __shared__ double sx[256], sy[256], sz[256];
+  const long long n = (long long)nz * ny * nx;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ix = (int)(c % nx), iy = (int)((c / nx) % ny), iz = (int)(c / ((long long)nx * ny));
+  const double cx = ((double)ix + 0.5) * hx, cy = ((double)iy + 0.5) * hy, cz = ((double)iz + 0.5) * hz;
+  double best = INFINITY;
+  int bi = 0;
+  for (int s0 = 0; s0 < ns; s0 += 256) {
+    __syncthreads();
+    if (s0 + (int)threadIdx.x < ns) {
+      sx[threadIdx.x] = seeds[3 * (s0 + threadIdx.x) + 0];
+      sy[threadIdx.x] = seeds[3 * (s0 + threadIdx.x) + 1];
+      sz[threadIdx.x] = seeds[3 * (s0 + threadIdx.x) + 2];
+    }
+    __syncthreads();
+    const int m = min(256, ns - s0);
+    for (int j = 0; j < m; ++j) {
+      const double dx = cx - sx[j], dy = cy - sy[j], dz = cz - sz[j];
+      const double d = dx * dx + dy * dy + dz * dz;
+      if (d < best) {
+        best = d;
+        bi = s0 + j;
+      }
+    }
+  }
+  if (c < n) out[c] = gori ? gori[bi] : bi;
 }
 
 // numpy.argmax semantics: first maximum; a NaN is the maximum (first NaN wins)
@@ -534,6 +569,19 @@ int pf_init_onehot(void* eta_dev, int32_t dtype, const int32_t* ori_dev, int64_t
   }
   hipError_t err = hipGetLastError();
   return err == hipSuccess ? PF_OK : hip_fail(err, "pf_init_onehot launch");
+}
+
+int pf_voronoi(const double* seeds_dev, int32_t n_seeds, const int32_t* grain_ori_dev, const pf_params_t* p,
+               int32_t* out_dev, void* stream) {
+  if (!seeds_dev || !out_dev || !p) return fail(PF_E_ARG, "NULL argument");
+  if (n_seeds < 1) return fail(PF_E_ARG, "n_seeds must be >= 1");
+  if (p->nz < 1 || p->ny < 1 || p->nx < 1) return fail(PF_E_ARG, "grid dimensions must be >= 1");
+  const long long n = (long long)p->nz * p->ny * p->nx;
+  if (!grid_ok(n, 1)) return fail(PF_E_ARG, "grid too large");
+  voronoi_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      seeds_dev, n_seeds, grain_ori_dev, p->nz, p->ny, p->nx, p->h[0], p->h[1], p->h[2], out_dev);
+  hipError_t err = hipGetLastError();
+  return err == hipSuccess ? PF_OK : hip_fail(err, "pf_voronoi launch");
 }
 
 int pf_argmax(const void* eta_dev, int32_t dtype, int64_t n_cells, int32_t k, int32_t* out_dev, void* stream) {

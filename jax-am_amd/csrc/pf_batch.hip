@@ -6,10 +6,10 @@
 //   - Instances are grouped into classes (dtype, K, anisotropy, tile shape).  Each
 //     class has a device-resident work list of (slot, tile) items walked by one
 //     persistent launch per step with a fixed grid.
-//   - `chunk` steps (class launches + advance) are captured into one HIP graph and
-//     replayed; admitting or removing an instance only rewrites table entries and
-//     work lists (stream-ordered copies), and re-captures only when a new class
-//     appears.
+//   - `chunk` steps (launches of the classes that have work + advance) are captured
+//     into one HIP graph and replayed.  Graphs are cached by (chunk, set of classes with
+//     work): admitting or removing an instance only rewrites table entries and work
+//     lists, and a new capture happens only for a class set not seen before.
 // Every tile of an instance runs exactly the per-cell arithmetic of pf_step with the
 // same tile shape, so an instance's trajectory does not depend on what else is in
 // the batch.
@@ -60,14 +60,18 @@ struct pf_batch {
   StepDev* d_desc = nullptr;
   BatchSlot* d_slots = nullptr;
   WorkItem* d_work = nullptr;
-  int* d_nwork = nullptr;
+  int2* d_info = nullptr;  // per class (work-list offset, count)
   std::vector<SlotHost> slots;
   std::vector<BatchClass> classes;
   bool work_dirty = false;
-  hipGraphExec_t graph = nullptr;
-  hipGraph_t graph_def = nullptr;
-  int graph_chunk = 0;
-  int graph_classes = -1;
+  struct Graph {
+    int chunk;
+    unsigned long long mask;
+    hipGraph_t def;
+    hipGraphExec_t exec;
+  };
+  static constexpr int kMaxGraphs = 16;
+  std::vector<Graph> graphs;  // most recently used last
   std::mutex mu;
 };
 
@@ -82,11 +86,11 @@ int rebuild_work(pf_batch* b) {
     for (int t = 0; t < sh.items; ++t) per[sh.cls].push_back(WorkItem{s, t});
   }
   std::vector<WorkItem> all;
-  std::vector<int> counts(pf_batch::kMaxClasses, 0);
+  std::vector<int2> info(pf_batch::kMaxClasses, make_int2(0, 0));
   for (size_t c = 0; c < b->classes.size(); ++c) {
     b->classes[c].off = (int)all.size();
     b->classes[c].count = (int)per[c].size();
-    counts[c] = (int)per[c].size();
+    info[c] = make_int2((int)all.size(), (int)per[c].size());
     all.insert(all.end(), per[c].begin(), per[c].end());
   }
   if ((int)all.size() > b->max_tiles) return fail(PF_E_ARG, "batch work list exceeds max_tiles");
@@ -94,7 +98,7 @@ int rebuild_work(pf_batch* b) {
   if (!all.empty())
     e = hipMemcpyAsync(b->d_work, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice, b->stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(b->d_nwork, counts.data(), counts.size() * sizeof(int), hipMemcpyHostToDevice, b->stream);
+    e = hipMemcpyAsync(b->d_info, info.data(), info.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream);
   // the host vectors above are pageable and local: finish the copies before they go away
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   if (e != hipSuccess) return hip_fail(e, "pf_batch work-list upload");
@@ -102,10 +106,18 @@ int rebuild_work(pf_batch* b) {
   return PF_OK;
 }
 
+unsigned long long active_mask(const pf_batch* b) {
+  unsigned long long m = 0;
+  for (size_t c = 0; c < b->classes.size(); ++c)
+    if (b->classes[c].count > 0) m |= 1ULL << c;
+  return m;
+}
+
 int launch_step_all(pf_batch* b) {
   for (size_t c = 0; c < b->classes.size(); ++c) {
     const BatchClass& k = b->classes[c];
-    hipError_t e = k.fn(b->d_desc, b->d_slots, b->d_work + k.off, b->d_nwork + c, k.grid, b->stream);
+    if (k.count == 0) continue;
+    hipError_t e = k.fn(b->d_desc, b->d_slots, b->d_work, b->d_info, (int)c, k.grid, b->stream);
     if (e != hipSuccess) return hip_fail(e, "pf_batch step launch");
   }
   advance_kernel<<<(b->max_inst + 255) / 256, 256, 0, b->stream>>>(b->d_slots, b->max_inst);
@@ -113,18 +125,26 @@ int launch_step_all(pf_batch* b) {
   return e == hipSuccess ? PF_OK : hip_fail(e, "pf_batch advance launch");
 }
 
-void drop_graph(pf_batch* b) {
-  if (b->graph) (void)hipGraphExecDestroy(b->graph);
-  if (b->graph_def) (void)hipGraphDestroy(b->graph_def);
-  b->graph = nullptr;
-  b->graph_def = nullptr;
-  b->graph_chunk = 0;
-  b->graph_classes = -1;
+void drop_graphs(pf_batch* b) {
+  for (pf_batch::Graph& g : b->graphs) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.def);
+  }
+  b->graphs.clear();
 }
 
-int ensure_graph(pf_batch* b, int chunk) {
-  if (b->graph && b->graph_chunk == chunk && b->graph_classes == (int)b->classes.size()) return PF_OK;
-  drop_graph(b);
+// The graph of `chunk` steps for the current class set (captured on first use).
+int get_graph(pf_batch* b, int chunk, hipGraphExec_t* out) {
+  const unsigned long long mask = active_mask(b);
+  for (size_t i = 0; i < b->graphs.size(); ++i) {
+    if (b->graphs[i].chunk == chunk && b->graphs[i].mask == mask) {
+      pf_batch::Graph g = b->graphs[i];
+      b->graphs.erase(b->graphs.begin() + i);
+      b->graphs.push_back(g);
+      *out = g.exec;
+      return PF_OK;
+    }
+  }
   hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
   int rc = PF_OK;
@@ -142,10 +162,15 @@ int ensure_graph(pf_batch* b, int chunk) {
     (void)hipGraphDestroy(g);
     return hip_fail(e, "hipGraphInstantiate");
   }
-  b->graph_def = g;
-  b->graph = ex;
-  b->graph_chunk = chunk;
-  b->graph_classes = (int)b->classes.size();
+  if ((int)b->graphs.size() >= pf_batch::kMaxGraphs) {
+    // the oldest graph may still be in flight: finish the stream before destroying it
+    (void)hipStreamSynchronize(b->stream);
+    (void)hipGraphExecDestroy(b->graphs.front().exec);
+    (void)hipGraphDestroy(b->graphs.front().def);
+    b->graphs.erase(b->graphs.begin());
+  }
+  b->graphs.push_back(pf_batch::Graph{chunk, mask, g, ex});
+  *out = ex;
   return PF_OK;
 }
 
@@ -166,14 +191,14 @@ pf_batch_t* pf_batch_create(int32_t max_instances, int32_t max_tiles, void* stre
   hipError_t e = hipMalloc((void**)&b->d_desc, sizeof(StepDev) * max_instances);
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_slots, sizeof(BatchSlot) * max_instances);
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_work, sizeof(WorkItem) * max_tiles);
-  if (e == hipSuccess) e = hipMalloc((void**)&b->d_nwork, sizeof(int) * pf_batch::kMaxClasses);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_info, sizeof(int2) * pf_batch::kMaxClasses);
   if (e == hipSuccess) {
     std::vector<BatchSlot> z(max_instances);
     memset(z.data(), 0, z.size() * sizeof(BatchSlot));
-    std::vector<int> zc(pf_batch::kMaxClasses, 0);
+    std::vector<int2> zc(pf_batch::kMaxClasses, make_int2(0, 0));
     e = hipMemcpyAsync(b->d_slots, z.data(), z.size() * sizeof(BatchSlot), hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(b->d_nwork, zc.data(), zc.size() * sizeof(int), hipMemcpyHostToDevice, b->stream);
+      e = hipMemcpyAsync(b->d_info, zc.data(), zc.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   }
   if (e != hipSuccess) {
@@ -187,11 +212,11 @@ pf_batch_t* pf_batch_create(int32_t max_instances, int32_t max_tiles, void* stre
 void pf_batch_destroy(pf_batch_t* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  drop_graph(b);
+  drop_graphs(b);
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_slots) (void)hipFree(b->d_slots);
   if (b->d_work) (void)hipFree(b->d_work);
-  if (b->d_nwork) (void)hipFree(b->d_nwork);
+  if (b->d_info) (void)hipFree(b->d_info);
   delete b;
 }
 
@@ -243,7 +268,7 @@ int pf_batch_add(pf_batch_t* b, const pf_batch_inst_t* inst) {
     if (k.dtype == inst->dtype && k.k == inst->p.num_oris && k.aniso == (int)aniso && k.tile == tile) cls = (int)c;
   }
   if (cls < 0) {
-    if ((int)b->classes.size() >= pf_batch::kMaxClasses) return fail(PF_E_STATE, "too many instance classes");
+    if ((int)b->classes.size() >= pf_batch::kMaxClasses) return fail(PF_E_STATE, "too many instance classes (64)");
     b->classes.push_back(BatchClass{inst->dtype, inst->p.num_oris, (int)aniso, tile, fn, grid, 0, 0});
     cls = (int)b->classes.size() - 1;
   }
@@ -293,10 +318,11 @@ int pf_batch_run(pf_batch_t* b, int32_t steps, int32_t chunk) {
   }
   int done = 0;
   if (chunk > 0 && steps >= chunk) {
-    int rc = ensure_graph(b, chunk);
+    hipGraphExec_t g = nullptr;
+    int rc = get_graph(b, chunk, &g);
     if (rc) return rc;
     for (; done + chunk <= steps; done += chunk) {
-      hipError_t e = hipGraphLaunch(b->graph, b->stream);
+      hipError_t e = hipGraphLaunch(g, b->stream);
       if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
     }
   }

@@ -27,6 +27,10 @@
 
 #include "pfhip.h"
 
+#ifndef PF_STORE_MODE
+#define PF_STORE_MODE 2  // 0: direct per-cell stores, 1: LDS-staged coalesced rows, 2: staged + non-temporal
+#endif
+
 namespace pf {
 
 // ------------------------------------------------------------------ storage types
@@ -51,6 +55,17 @@ template <> struct Traits<bf16_t> {
 template <class T, int N> struct alignas(sizeof(T) * N) Vec {
   T v[N];
 };
+
+// non-temporal vector store of a 2/4/8/16-byte value
+template <int BYTES> struct RawT;
+template <> struct RawT<16> { typedef unsigned int type __attribute__((ext_vector_type(4))); };
+template <> struct RawT<8> { typedef unsigned int type __attribute__((ext_vector_type(2))); };
+template <> struct RawT<4> { typedef unsigned int type; };
+template <> struct RawT<2> { typedef unsigned short type; };
+template <class V> __device__ __forceinline__ void store_nt(V* p, const V& v) {
+  using R = typename RawT<sizeof(V)>::type;
+  __builtin_nontemporal_store(__builtin_bit_cast(R, v), (R*)p);
+}
 
 __device__ __forceinline__ float to_tc(float x) { return x; }
 __device__ __forceinline__ double to_tc(double x) { return x; }
@@ -88,6 +103,7 @@ struct StepDev {
   double m_over_ad, kappa_ad, gamma4, B;  // m_g/ad_hoc, kappa_g*ad_hoc, 4*gamma, 1-4*gamma
   double L0, Qg, gas_const, T_l, T_clip, theta, zeta_window, eps, liquid_T;
   double t_prev, hstep;
+  double stiff;  // m_g/ad_hoc (4 + 4|gamma|) + 4 kappa_g ad_hoc (1 + eps) sum 1/h^2 (cold-path threshold)
   // laser
   double T_amb, p0x, p0y, p0z, dirx, diry, dirz, vel;
   double q_coef, pe, T_cap;          // Rosenthal: Q/(2 pi kappa_T), vel/(2 alpha)
@@ -130,9 +146,85 @@ template <class TC> struct CellCoef {
   bool liquid;
 };
 
-// Per-cell coefficients of the update (allen_cahn.py:55-58,83-84,124-126).
+// ---- FP32 temperature for the cold-cell path
+// Launch-uniform FP32 constants of the temperature model at time t and the FP64/FP32
+// path threshold.  Built once per thread (a few FP64 operations).
+struct ColdPath {
+  float px, py, pz;        // source position at t
+  float hx, hy, hz;
+  float T_floor;           // cells with T < T_floor take the FP32 mobility path
+  float L0, Qg_R;          // L0, Qg / gas_const (Arrhenius)
+  float hm, hk, eps1, eps2;   // hstep*m_g/ad_hoc, hstep*kappa_g*ad_hoc, 1+eps, 2*eps
+  float gamma4;
+  float T_amb, dT, gxy, gz, q, pe, dxr, dyr, dzr, T_cap, T_ref, grad_T, cool_t;
+};
+
+// T_floor: the temperature below which the mobility is so small that its FP32 evaluation
+// (relative error ~1e-6) changes an update by less than 1e-12:  L(T_floor) * hstep * stiff
+// = 1e-6, stiff = m_g/ad_hoc (4 + 4 gamma) + 4 kappa_g ad_hoc (1 + eps) sum 1/h^2.  Kept at
+// least 10 K below T_liquidus, the liquid-reset threshold and T_clip, so the cold path's
+// zeta = 1 and no-reset are the exact FP64 decisions.
+__device__ __forceinline__ ColdPath make_cold(const StepDev& a, double t) {
+  ColdPath k;
+  const double vt = a.vel * t;
+  k.px = (float)(a.p0x + vt * a.dirx);
+  k.py = (float)(a.p0y + vt * a.diry);
+  k.pz = (float)(a.p0z + vt * a.dirz);
+  k.hx = (float)a.hx;
+  k.hy = (float)a.hy;
+  k.hz = (float)a.hz;
+  double tf = -INFINITY;
+  const double sc = a.hstep * a.stiff;
+  if (sc > 0.0 && a.L0 > 0.0 && a.Qg > 0.0 && a.gas_const > 0.0) {
+    const double Lfl = 1e-6 / sc;
+    tf = Lfl < a.L0 ? -a.Qg / (a.gas_const * log(Lfl / a.L0)) : INFINITY;
+    tf = fmin(tf, fmin(a.T_l, fmin(a.liquid_T, a.T_clip)) - 10.0);
+  }
+  k.T_floor = (float)tf;
+  k.L0 = (float)a.L0;
+  k.Qg_R = (float)(a.Qg / a.gas_const);
+  k.hm = (float)(a.hstep * a.m_over_ad);
+  k.hk = (float)(a.hstep * a.kappa_ad);
+  k.eps1 = (float)(1.0 + a.eps);
+  k.eps2 = (float)(2.0 * a.eps);
+  k.gamma4 = (float)a.gamma4;
+  k.T_amb = (float)a.T_amb;
+  k.dT = (float)a.dT_peak;
+  k.gxy = (float)a.g_sxy;
+  k.gz = (float)a.g_sz;
+  k.q = (float)a.q_coef;
+  k.pe = (float)a.pe;
+  k.dxr = (float)a.dirx;
+  k.dyr = (float)a.diry;
+  k.dzr = (float)a.dirz;
+  k.T_cap = (float)a.T_cap;
+  k.T_ref = (float)a.T_ref;
+  k.grad_T = (float)a.grad_T;
+  k.cool_t = (float)(a.cool_rate * t);
+  return k;
+}
+
+__device__ __forceinline__ float cell_temperature_f32(const StepDev& a, const ColdPath& k, int ix, int iy,
+                                                      int iz, long long c) {
+  if (a.tmodel == PF_T_BUFFER) return a.t_f64 ? (float)((const double*)a.T)[c] : ((const float*)a.T)[c];
+  const float cz = ((float)iz + 0.5f) * k.hz;
+  if (a.tmodel == PF_T_QUENCH) return k.T_ref + k.grad_T * cz - k.cool_t;
+  const float X = ((float)ix + 0.5f) * k.hx - k.px;
+  const float Y = ((float)iy + 0.5f) * k.hy - k.py;
+  const float Z = cz - k.pz;
+  if (a.tmodel == PF_T_ROSENTHAL) {
+    const float R = sqrtf(X * X + Y * Y + Z * Z);
+    const float proj = X * k.dxr + Y * k.dyr + Z * k.dzr;
+    float T = k.T_amb + k.q / R * expf(-k.pe * (R + proj));
+    if (k.T_cap > 0.0f && T > k.T_cap) T = k.T_cap;
+    return T;
+  }
+  return k.T_amb + k.dT * expf(-((X * X + Y * Y) * k.gxy + Z * Z * k.gz));
+}
+
+// Per-cell coefficients of the update (allen_cahn.py:55-58,83-84,124-126), FP64 evaluation.
 template <class TC>
-__device__ __forceinline__ CellCoef<TC> cell_coef(const StepDev& a, double T, TC S) {
+__device__ __forceinline__ CellCoef<TC> cell_coef64(const StepDev& a, double T, TC S) {
   CellCoef<TC> r;
   const double Tc = T > a.T_clip ? a.T_clip : T;                 // :56
   // :57  zeta = 0.5*(1 - tanh(theta*(Tc/T_l - 1))).  In FP64, tanh(x) rounds to exactly +-1
@@ -161,14 +253,55 @@ __device__ __forceinline__ CellCoef<TC> cell_coef(const StepDev& a, double T, TC
   return r;
 }
 
+// Per-cell coefficients.  F64 storage: always the FP64 evaluation.  FP32 arithmetic: each
+// cell whose FP32 temperature is below ColdPath::T_floor takes an FP32 mobility (zeta = 1,
+// no liquid reset: exact there); every other cell takes the FP64 evaluation.  The choice is
+// made per cell from the cell's own temperature, so results do not depend on which cells
+// share a wavefront; the FP64 branch is entered only by wavefronts holding a hot cell.
+template <class TC>
+__device__ __forceinline__ CellCoef<TC> cell_coef(const StepDev& a, const ColdPath& k, int ix, int iy, int iz,
+                                                  long long c, double t, TC S) {
+  if constexpr (sizeof(TC) == 8) {
+    (void)k;
+    return cell_coef64<TC>(a, cell_temperature(a, ix, iy, iz, c, t), S);
+  } else {
+    const float T32 = cell_temperature_f32(a, k, ix, iy, iz, c);
+    const bool hot = !(T32 < k.T_floor);  // NaN -> FP64 path
+    CellCoef<float> r;
+    const float Lg = k.L0 * expf(-k.Qg_R / T32);
+    r.c1 = k.hm * Lg;
+    r.c2 = k.hk * Lg;
+    r.w1 = r.c2 * k.eps1;
+    r.w2 = r.c2 * k.eps2;
+    r.A = fmac(k.gamma4, S, -1.0f);
+    r.liquid = false;
+    if (__ballot(hot)) {
+      if (hot) r = cell_coef64<float>(a, cell_temperature(a, ix, iy, iz, c, t), S);
+    }
+    return r;
+  }
+}
+
+// Bitwise "non-zero" test of a channel's 7-point stencil (a -0.0 counts as non-zero).
+__device__ __forceinline__ bool any_nz(float c, float a, float b, float d, float e, float f, float g) {
+  return (__float_as_uint(c) | __float_as_uint(a) | __float_as_uint(b) | __float_as_uint(d) | __float_as_uint(e) |
+          __float_as_uint(f) | __float_as_uint(g)) != 0u;
+}
+__device__ __forceinline__ bool any_nz(double c, double a, double b, double d, double e, double f, double g) {
+  return (__double_as_longlong(c) | __double_as_longlong(a) | __double_as_longlong(b) | __double_as_longlong(d) |
+          __double_as_longlong(e) | __double_as_longlong(f) | __double_as_longlong(g)) != 0;
+}
+
 // One orientation channel: centre value c, x/y/z neighbours (edge-replicated at the
-// domain faces by the caller), returns eta_new.
-// With RHS = true it returns -L*(grad + laplace) instead (c1, c2 then carry no dt and the
-// liquid reset is not applied), the reference's state_rhs value.
+// domain faces by the caller), returns eta_new before the liquid reset.
+// With RHS = true it returns -L*(grad + laplace) instead (c1, c2 then carry no dt),
+// the reference's state_rhs value.
+// The anisotropy factor is skipped when |grad eta|^2 == 0 on every lane of the wavefront:
+// there the full evaluation gives q = 1 and ca = w1 exactly, which is what the skip uses.
 template <class TC, bool ANISO, bool RHS = false>
 __device__ __forceinline__ TC update_one(const CellCoef<TC>& cf, TC B, TC ihx2, TC ihy2, TC ihz2,
                                          TC c, TC xm, TC xp, TC ym, TC yp, TC zm, TC zp,
-                                         const TC* __restrict__ ax /* 9 values: axis k, comp d at [3k+d] */) {
+                                         const TC* __restrict__ axes, int K, int i) {
   const TC dx = fmac(c, (TC)-2, xm + xp);
   const TC dy = fmac(c, (TC)-2, ym + yp);
   const TC dz = fmac(c, (TC)-2, zm + zp);
@@ -177,20 +310,29 @@ __device__ __forceinline__ TC update_one(const CellCoef<TC>& cf, TC B, TC ihx2, 
   if constexpr (ANISO) {
     const TC ex = xp - xm, ey = yp - ym, ez = zp - zm;
     const TC n2 = fmac(ez, ez, fmac(ey, ey, ex * ex));
-    const TC p0 = fmac(ax[2], ez, fmac(ax[1], ey, ax[0] * ex));
-    const TC p1 = fmac(ax[5], ez, fmac(ax[4], ey, ax[3] * ex));
-    const TC p2 = fmac(ax[8], ez, fmac(ax[7], ey, ax[6] * ex));
-    TC m;
-    TC q;
-    if constexpr (sizeof(TC) == 4) {
-      m = fmaxf(fmaxf(fabsf(p0), fabsf(p1)), fabsf(p2));
-      q = fminf(m * m * rcp_approx(n2), 1.0f);   // n2 == 0 -> NaN -> 1 (minNum)
-    } else {
-      m = fmax(fmax(fabs(p0), fabs(p1)), fabs(p2));
-      q = n2 > 0.0 ? fmin((m * m) / n2, 1.0) : 1.0;
+    ca = cf.w1;
+    if (__ballot(n2 != (TC)0)) {
+      // axes [3][K][3]: axis k of orientation i at (k*K + i)*3
+      const TC* a0 = axes + i * 3;
+      const TC* a1 = axes + (K + i) * 3;
+      const TC* a2 = axes + (2 * K + i) * 3;
+      const TC p0 = fmac(a0[2], ez, fmac(a0[1], ey, a0[0] * ex));
+      const TC p1 = fmac(a1[2], ez, fmac(a1[1], ey, a1[0] * ex));
+      const TC p2 = fmac(a2[2], ez, fmac(a2[1], ey, a2[0] * ex));
+      TC q;
+      if constexpr (sizeof(TC) == 4) {
+        const TC m = fmaxf(fmaxf(fabsf(p0), fabsf(p1)), fabsf(p2));
+        q = fminf(m * m * rcp_approx(n2), 1.0f);   // n2 == 0 -> NaN -> 1 (minNum)
+      } else {
+        const TC m = fmax(fmax(fabs(p0), fabs(p1)), fabs(p2));
+        q = n2 > 0.0 ? fmin((m * m) / n2, 1.0) : 1.0;
+      }
+      ca = fmac(cf.w2 * q, q - (TC)1, cf.w1);
     }
-    ca = fmac(cf.w2 * q, q - (TC)1, cf.w1);
   } else {
+    (void)axes;
+    (void)K;
+    (void)i;
     ca = cf.c2;
   }
   const TC e2 = c * c;
@@ -200,72 +342,111 @@ __device__ __forceinline__ TC update_one(const CellCoef<TC>& cf, TC B, TC ihx2, 
     return fmac(ca, lap, -cf.c1 * u);
   } else {
     const TC r = fmac(-cf.c1, u, c);
-    const TC v = fmac(ca, lap, r);
-    return cf.liquid ? (TC)0 : v;
+    return fmac(ca, lap, r);
   }
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // ------------------------------------------------------------------ tile geometry
-template <class TS, int K> struct Layout {
+constexpr int p2div(int n, int cap) {
+  int v = cap;
+  while (v > 1 && n % v) v /= 2;
+  return v;
+}
+
+// Channel split of the sum of squares S: for even K >= 16 the register-tiled kernels hold
+// the channels of a cell in two halves (two threads), and every kernel sums S as
+// (sum of the first K/2 squares) + (sum of the last K/2 squares), so all kernels of a K
+// round identically.
+constexpr int split_of(int K) { return K < 0 ? 2 : 1; }  // the two-half split is disabled: one thread per cell
+
+// H threads per cell (H = 2: each of two wavefronts holds half of the K channels of the
+// same 64 cells), KH = K / H channels per thread.
+template <class TS, int K, int H> struct Layout {
   using TC = typename Traits<TS>::TC;
-  // elements per global load chunk (must divide K; at most 16 bytes of storage and 16 bytes of TC)
-  static constexpr int VEC_MAX = (16 / (int)sizeof(TC)) < (16 / (int)sizeof(TS)) ? (16 / (int)sizeof(TC))
-                                                                                   : (16 / (int)sizeof(TS));
-  static constexpr int VEC = (K % VEC_MAX == 0) ? VEC_MAX
-                             : (VEC_MAX >= 2 && K % 2 == 0) ? 2 : 1;
-  // LDS cell stride: 16-byte groups, odd number of groups per cell (bank-conflict-free b128 reads)
-  static constexpr int LV = 16 / (int)sizeof(TC);
-  static constexpr int KG = (K + LV - 1) / LV;
-  static constexpr int KP = (KG % 2 == 1 ? KG : KG + 1) * LV;
-  static constexpr int NG = KG;  // groups read per neighbour
+  static_assert(K % H == 0, "H must divide K");
+  static constexpr int KH = K / H;
+  // global load chunk (elements): a power of two dividing K, <= 16 bytes of TS and of TC
+  static constexpr int VEC = p2div(K, (16 / (int)sizeof(TC)) < (16 / (int)sizeof(TS)) ? 16 / (int)sizeof(TC)
+                                                                                        : 16 / (int)sizeof(TS));
+  // LDS read vector of a thread's channel slice (TC elements), and the cell stride KP: a
+  // multiple of LV with KP / LV odd, so consecutive lanes' LV-wide reads hit distinct banks
+  static constexpr int LV = p2div(KH, 16 / (int)sizeof(TC));
+  static constexpr int KP0 = (K + LV - 1) / LV;
+  static constexpr int KP = (KP0 % 2 == 1 ? KP0 : KP0 + 1) * LV;
+  static constexpr int NG = KH / LV;
+  static constexpr int WV = p2div(KP, VEC);  // LDS write vector of a load chunk (divides KP and VEC)
+  // store vector of a thread's slice (TS elements)
+  static constexpr int SVS = p2div(KH, 16 / (int)sizeof(TS));
 };
 
-template <class TS, int K, int TX, int TY> struct Tile {
-  using L = Layout<TS, K>;
+template <class TS, int K, int TX, int TY, int H> struct Tile {
+  using L = Layout<TS, K, H>;
   using TC = typename L::TC;
-  static constexpr int NT = TX * TY;
+  static constexpr int NC = TX * TY;             // cells per tile
+  static constexpr int NT = NC * H;              // threads per workgroup
+  static_assert(H == 1 || NC % 64 == 0, "H = 2 needs whole wavefronts of cells");
   static constexpr int PX = TX + 2, PY = TY + 2;
   static constexpr int CPC = K / L::VEC;         // chunks per cell
   static constexpr int NCH = PY * PX * CPC;      // chunks per plane
   static constexpr int CPT = (NCH + NT - 1) / NT;
   static constexpr int PLANE = PY * PX * L::KP;  // TC elements per LDS plane
-  static constexpr size_t LDS_BYTES = 2 * PLANE * sizeof(TC);
+  static constexpr int STAGE = PF_STORE_MODE != 0 ? NC * K : 8;  // TS elements of output staging
+  static constexpr int SRING = H > 1 ? 2 * NC * H : 1;          // partial sums of squares, 2 planes
+  static constexpr size_t LDS_BYTES = 2 * PLANE * sizeof(TC) + STAGE * sizeof(TS) + SRING * sizeof(TC);
 };
 
 // ------------------------------------------------------------------ 2.5-D march of one tile
-// A workgroup of TX*TY threads owns a TX x TY column of cells and marches it
-// upward through z-planes [zb, ze).  Planes (tile + one-cell halo ring,
-// edge-replicated at the domain faces) are double-buffered in LDS; the plane two
-// ahead is prefetched into registers while the current plane is computed.  Each
-// thread keeps the K channels of its own cell at z-1, z, z+1 in three register
-// arrays whose roles rotate by renaming (the z loop is unrolled by three).
-template <int K, class TS, bool ANISO, int TX, int TY>
+// A workgroup owns a TX x TY column of cells and marches it upward through z-planes
+// [zb, ze).  Planes (tile + one-cell halo ring, edge-replicated at the domain faces) are
+// double-buffered in LDS; the plane two ahead is prefetched into registers while the
+// current plane is computed.  Each thread keeps its KH channels of its cell at z-1, z, z+1
+// in three register arrays whose roles rotate by renaming (the z loop is unrolled by
+// three).  With H = 2 the two halves of a cell's channels live in two wavefronts; the sum
+// of squares S = sum_j eta_j^2 is completed through an LDS ring written one plane ahead,
+// so it needs no extra barrier.  Results are staged in LDS and written back as contiguous
+// row segments (PF_STORE_MODE 1, 2) or stored per cell (0).
+template <int K, class TS, bool ANISO, int TX, int TY, int H>
 struct March {
   using TC = typename Traits<TS>::TC;
-  using L = Layout<TS, K>;
-  using G = Tile<TS, K, TX, TY>;
-  static constexpr int VEC = L::VEC, KP = L::KP, LV = L::LV, NG = L::NG;
+  using L = Layout<TS, K, H>;
+  using G = Tile<TS, K, TX, TY, H>;
+  static constexpr int VEC = L::VEC, KP = L::KP, LV = L::LV, NG = L::NG, KH = L::KH, WV = L::WV;
   using SV = Vec<TS, VEC>;
-  using CV = Vec<TC, VEC>;
+  using WVv = Vec<TC, WV>;
   using LVv = Vec<TC, LV>;
+  using SVv = Vec<TS, L::SVS>;
 
   const StepDev a;
   TC* __restrict__ buf0;
   TC* __restrict__ buf1;
   const TS* __restrict__ in;
   TS* __restrict__ out;
+  TS* __restrict__ stage;  // LDS staging of one output plane (row-major cells, K each)
+  TC* __restrict__ sring;  // LDS [2][H][NC] partial sums of squares
   int src_off[G::CPT];
   int dst_off[G::CPT];
   SV pre[G::CPT];
-  int gx, gy, own, oxm, oxp, oym, oyp, zb, len;
+  int cl, h, ch0, gx, gy, own, oxm, oxp, oym, oyp, zb, len;
   long long plane_elems;
   bool active;
+  ColdPath cold;
 
-  __device__ __forceinline__ March(const StepDev& a_, int tile_x, int tile_y, int tile_z, TC* smem) : a(a_) {
+  __device__ __forceinline__ March(const StepDev& a_, int tile_x, int tile_y, int tile_z, TC* smem, TS* stage_,
+                                   TC* sring_)
+      : a(a_), cold(make_cold(a_, a_.t_prev)) {
     const int tid = threadIdx.x;
-    const int lx = tid % TX, ly = tid / TX;
+    if constexpr (H == 1) {
+      cl = tid;
+      h = 0;
+    } else {
+      const int w = tid >> 6;
+      h = w % H;
+      cl = (w / H) * 64 + (tid & 63);
+    }
+    ch0 = h * KH;
+    const int lx = cl % TX, ly = cl / TX;
     gx = tile_x * TX + lx;
     gy = tile_y * TY + ly;
     active = (gx < a.nx) && (gy < a.ny);
@@ -276,6 +457,8 @@ struct March {
     out = (TS*)a.out;
     buf0 = smem;
     buf1 = smem + G::PLANE;
+    stage = stage_;
+    sring = sring_;
 #pragma unroll
     for (int j = 0; j < G::CPT; ++j) {
       const int w = tid + j * G::NT;
@@ -287,7 +470,7 @@ struct March {
       src_off[j] = (sy * a.nx + sx) * K + v * VEC;
       dst_off[j] = (w < G::NCH) ? cs * KP + v * VEC : -1;
     }
-    own = ((ly + 1) * G::PX + (lx + 1)) * KP;
+    own = ((ly + 1) * G::PX + (lx + 1)) * KP + ch0;
     oxm = own - KP;
     oxp = own + KP;
     oym = own - G::PX * KP;
@@ -303,39 +486,59 @@ struct March {
 #pragma unroll
     for (int j = 0; j < G::CPT; ++j) {
       if (dst_off[j] >= 0) {
-        CV c;
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) c.v[e] = to_tc(pre[j].v[e]);
-        *(CV*)(b + dst_off[j]) = c;
+        for (int q = 0; q < VEC / WV; ++q) {
+          WVv c;
+#pragma unroll
+          for (int e = 0; e < WV; ++e) c.v[e] = to_tc(pre[j].v[q * WV + e]);
+          *(WVv*)(b + dst_off[j] + q * WV) = c;
+        }
       }
     }
   }
-  __device__ __forceinline__ void read_center(const TC* b, TC (&dst)[K]) {
+  __device__ __forceinline__ void read_center(const TC* b, TC (&dst)[KH]) {
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
       const LVv v = *(const LVv*)(b + own + g * LV);
 #pragma unroll
-      for (int l = 0; l < LV; ++l)
-        if (g * LV + l < K) dst[g * LV + l] = v.v[l];
+      for (int l = 0; l < LV; ++l) dst[g * LV + l] = v.v[l];
+    }
+  }
+  __device__ __forceinline__ TC part_sq(const TC (&c)[KH]) {
+    TC S = 0;
+#pragma unroll
+    for (int i = 0; i < KH; ++i) S = fmac(c[i], c[i], S);
+    return S;
+  }
+  // the partial sum of squares of this thread's slice of plane zb+n, for the other half
+  __device__ __forceinline__ void put_sq(int n, TC s) {
+    if constexpr (H > 1) sring[((n & 1) * H + h) * G::NC + cl] = s;
+  }
+  // S of plane zb+n: own partial + the other halves' partials, in channel order
+  __device__ __forceinline__ TC full_sq(int n, TC own_part) {
+    if constexpr (H == 1) {
+      return own_part;
+    } else {
+      TC S = 0;
+#pragma unroll
+      for (int q = 0; q < H; ++q) S += (q == h) ? own_part : sring[((n & 1) * H + q) * G::NC + cl];
+      return S;
     }
   }
 
   // Compute plane zb+n from (lo, c, hi) = planes z-1, z, z+1, then (if another plane
   // follows) rotate the LDS ring and read plane z+2 into `lo`.
-  __device__ __forceinline__ void body(int n, TC (&lo)[K], TC (&c)[K], TC (&hi)[K]) {
+  __device__ __forceinline__ void body(int n, TC (&lo)[KH], TC (&c)[KH], TC (&hi)[KH]) {
     const int z = zb + n;
     TC* cb = (n & 1) ? buf1 : buf0;
     if (active) {
       const long long cell = ((long long)z * a.ny + gy) * a.nx + gx;
-      TC S = 0;
-#pragma unroll
-      for (int i = 0; i < K; ++i) S = fmac(c[i], c[i], S);
-      const double T = cell_temperature(a, gx, gy, z, cell, a.t_prev);
-      const CellCoef<TC> cf = cell_coef<TC>(a, T, S);
+      const TC S = full_sq(n, part_sq(c));
+      const CellCoef<TC> cf = cell_coef<TC>(a, cold, gx, gy, z, cell, a.t_prev, S);
       const TC B = (TC)a.B, ihx2 = (TC)a.ihx2, ihy2 = (TC)a.ihy2, ihz2 = (TC)a.ihz2;
       const TC* __restrict__ axes = (const TC*)a.axes;
       if constexpr (ANISO) asm volatile("" : "+s"(axes));  // keep the axis table out of VGPRs: reload per plane
-      TC res[K];
+      TC res[KH];
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         const LVv vxm = *(const LVv*)(cb + oxm + g * LV);
@@ -345,39 +548,78 @@ struct March {
 #pragma unroll
         for (int l = 0; l < LV; ++l) {
           const int i = g * LV + l;
-          if (i < K) {
-            TC ax[9];
-            if constexpr (ANISO) {
-#pragma unroll
-              for (int kk = 0; kk < 3; ++kk)
-#pragma unroll
-                for (int d = 0; d < 3; ++d) ax[3 * kk + d] = axes[(kk * K + i) * 3 + d];
-            }
+          // A channel that is exactly zero on the whole stencil of every lane stays
+          // exactly +0 (the full update evaluates to +0 there); skip it wave-wide.
+          if (__ballot(any_nz(c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l], lo[i], hi[i])))
             res[i] = update_one<TC, ANISO>(cf, B, ihx2, ihy2, ihz2, c[i], vxm.v[l], vxp.v[l], vym.v[l],
-                                           vyp.v[l], lo[i], hi[i], ax);
-          }
+                                           vyp.v[l], lo[i], hi[i], axes, K, ch0 + i);
+          else
+            res[i] = (TC)0;
         }
       }
-      TS* o = out + cell * K;
+      if (__ballot(cf.liquid)) {
 #pragma unroll
-      for (int j = 0; j < K / VEC; ++j) {
-        SV s;
+        for (int i = 0; i < KH; ++i) res[i] = cf.liquid ? (TC)0 : res[i];
+      }
+#if PF_STORE_MODE == 0
+      TS* o = out + cell * K + ch0;
+#else
+      TS* o = stage + cl * K + ch0;
+#endif
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) s.v[e] = from_tc<TS>(res[j * VEC + e]);
-        *(SV*)(o + j * VEC) = s;
+      for (int j = 0; j < KH / L::SVS; ++j) {
+        SVv s;
+#pragma unroll
+        for (int e = 0; e < L::SVS; ++e) s.v[e] = from_tc<TS>(res[j * L::SVS + e]);
+        *(SVv*)(o + j * L::SVS) = s;
       }
     }
+    __syncthreads();
+#if PF_STORE_MODE != 0
+    store_rows(z);
+#endif
     if (n + 1 < len) {
-      __syncthreads();
       store_plane(cb);  // plane z+2 replaces plane z
       if (n + 2 < len) load_plane(min(z + 3, a.nz - 1));
       __syncthreads();
       read_center(cb, lo);
+      put_sq(n, part_sq(lo));  // plane z+2 uses ring slot n & 1 (its last reader was plane z)
+    }
+  }
+
+  // Coalesced write-back of plane z of the tile from the LDS staging rows: consecutive
+  // lanes store consecutive CH-byte pieces of each contiguous (y, z) row segment.
+  __device__ __forceinline__ void store_rows(int z) {
+    constexpr int CELLB = K * (int)sizeof(TS);
+    constexpr int CH = (CELLB & -CELLB) >= 16 ? 16 : (CELLB & -CELLB);
+    using CT = typename RawT<CH>::type;
+    constexpr int ROWB = TX * CELLB;
+    constexpr int CPR = ROWB / CH;
+    constexpr int NCHO = TY * CPR;
+    const int x0 = gx - cl % TX;
+    const int y0 = gy - cl / TX;
+    const int rowb = min(TX, a.nx - x0) * CELLB;
+#pragma unroll
+    for (int j = 0; j < (NCHO + G::NT - 1) / G::NT; ++j) {
+      const int w = threadIdx.x + j * G::NT;
+      if (w < NCHO) {
+        const int r = w / CPR;
+        const int off = (w - r * CPR) * CH;
+        if (y0 + r < a.ny && off < rowb) {
+          const CT v = *(const CT*)((const char*)stage + r * ROWB + off);
+          char* dst = (char*)(out + (((long long)z * a.ny + y0 + r) * a.nx + x0) * K) + off;
+#if PF_STORE_MODE == 2
+          __builtin_nontemporal_store(v, (CT*)dst);
+#else
+          *(CT*)dst = v;
+#endif
+        }
+      }
     }
   }
 
   __device__ __forceinline__ void run() {
-    TC r0[K], r1[K], r2[K];
+    TC r0[KH], r1[KH], r2[KH];
     // prologue: planes zb, zb+1 into LDS; plane zb-1 (edge-replicated) straight into registers
     load_plane(zb);
     store_plane(buf0);
@@ -386,18 +628,21 @@ struct March {
     {
       const int zbh = max(zb - 1, 0);
       const int sxo = min(gx, a.nx - 1), syo = min(gy, a.ny - 1);
-      const TS* p = in + (long long)zbh * plane_elems + (long long)(syo * a.nx + sxo) * K;
+      const TS* p = in + (long long)zbh * plane_elems + (long long)(syo * a.nx + sxo) * K + ch0;
 #pragma unroll
-      for (int j = 0; j < K / VEC; ++j) {
-        const SV s = *(const SV*)(p + j * VEC);
+      for (int j = 0; j < KH / L::SVS; ++j) {
+        const SVv s = *(const SVv*)(p + j * L::SVS);
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) r0[j * VEC + e] = to_tc(s.v[e]);
+        for (int e = 0; e < L::SVS; ++e) r0[j * L::SVS + e] = to_tc(s.v[e]);
       }
     }
     __syncthreads();
     read_center(buf0, r1);
     read_center(buf1, r2);
+    put_sq(0, part_sq(r1));
+    put_sq(1, part_sq(r2));
     if (len > 1) load_plane(min(zb + 2, a.nz - 1));
+    __syncthreads();
     for (int n = 0; n < len; n += 3) {
       body(n, r0, r1, r2);
       if (n + 1 < len) body(n + 1, r1, r2, r0);
@@ -406,10 +651,11 @@ struct March {
   }
 };
 
-template <int K, class TS, bool ANISO, int TX, int TY>
+template <int K, class TS, bool ANISO, int TX, int TY, int H>
 __device__ __forceinline__ void march_tile(const StepDev& a, int tile_x, int tile_y, int tile_z,
-                                           typename Traits<TS>::TC* __restrict__ smem) {
-  March<K, TS, ANISO, TX, TY> m(a, tile_x, tile_y, tile_z, smem);
+                                           typename Traits<TS>::TC* __restrict__ smem, TS* __restrict__ stage,
+                                           typename Traits<TS>::TC* __restrict__ sring) {
+  March<K, TS, ANISO, TX, TY, H> m(a, tile_x, tile_y, tile_z, smem, stage, sring);
   m.run();
 }
 
@@ -431,25 +677,30 @@ __device__ __forceinline__ void generic_cell(const StepDev& a, long long cell) {
   const long long oxm = ix > 0 ? -sx : 0, oxp = ix < nx - 1 ? sx : 0;
   const long long oym = iy > 0 ? -sy : 0, oyp = iy < ny - 1 ? sy : 0;
   const long long ozm = iz > 0 ? -sz : 0, ozp = iz < nz - 1 ? sz : 0;
+  const int H = split_of(K), KH = K / H;
   TC S = 0;
-  for (int i = 0; i < K; ++i) {
-    const TC c = to_tc(in[base + i]);
-    S = fmac(c, c, S);
+  for (int q = 0; q < H; ++q) {
+    TC Sq = 0;
+    for (int i = q * KH; i < (q + 1) * KH; ++i) {
+      const TC c = to_tc(in[base + i]);
+      Sq = fmac(c, c, Sq);
+    }
+    S += Sq;
   }
-  const double T = cell_temperature(a, ix, iy, iz, cell, a.t_prev);
-  const CellCoef<TC> cf = cell_coef<TC>(a, T, S);
+  const ColdPath k = make_cold(a, a.t_prev);
+  const CellCoef<TC> cf = cell_coef<TC>(a, k, ix, iy, iz, cell, a.t_prev, S);
   const TC B = (TC)a.B, ihx2 = (TC)a.ihx2, ihy2 = (TC)a.ihy2, ihz2 = (TC)a.ihz2;
   const TC* __restrict__ axes = (const TC*)a.axes;
   for (int i = 0; i < K; ++i) {
     const long long e = base + i;
-    TC ax[9];
-    if constexpr (ANISO) {
-      for (int kk = 0; kk < 3; ++kk)
-        for (int d = 0; d < 3; ++d) ax[3 * kk + d] = axes[(kk * K + i) * 3 + d];
-    }
-    const TC v = update_one<TC, ANISO, RHS>(cf, B, ihx2, ihy2, ihz2, to_tc(in[e]), to_tc(in[e + oxm]),
-                                       to_tc(in[e + oxp]), to_tc(in[e + oym]), to_tc(in[e + oyp]),
-                                       to_tc(in[e + ozm]), to_tc(in[e + ozp]), ax);
+    const TC c = to_tc(in[e]), xm = to_tc(in[e + oxm]), xp = to_tc(in[e + oxp]), ym = to_tc(in[e + oym]),
+             yp = to_tc(in[e + oyp]), zm = to_tc(in[e + ozm]), zp = to_tc(in[e + ozp]);
+    TC v;
+    if (RHS || __ballot(any_nz(c, xm, xp, ym, yp, zm, zp)))
+      v = update_one<TC, ANISO, RHS>(cf, B, ihx2, ihy2, ihz2, c, xm, xp, ym, yp, zm, zp, axes, K, i);
+    else
+      v = (TC)0;
+    if (!RHS && cf.liquid) v = (TC)0;
     out[e] = from_tc<TS>(v);
   }
 }

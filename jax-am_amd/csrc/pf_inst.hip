@@ -2,7 +2,7 @@
 //   -DPF_INST_DTYPE=<PF_F64|PF_F32|PF_BF16 value> -DPF_KLO=<k> -DPF_KHI=<k> -DPF_PART=<n>
 // (see Makefile) so the instantiations build in parallel.  K = 1..24 and 32 have
 // register-tiled kernels; other K run on the generic kernel.
-// Tile shapes (TX, TY): F32/BF16: 0 = 32x8, 1 = 64x4, 2 = 32x4;  F64: 0 = 32x4, 1 = 16x4.
+// Tile shapes (TX, TY): F32/BF16: 0 = 32x8, 1 = 64x4, 2 = 16x8;  F64: 0 = 32x4, 1 = 16x4.
 #include "pf_kernels.h"
 
 #if !defined(PF_INST_DTYPE) || !defined(PF_KLO) || !defined(PF_KHI) || !defined(PF_PART)
@@ -15,13 +15,13 @@
 #if PF_INST_DTYPE == 1
 #define PF_K_ENTRIES(K)                                                                          \
   PF_ENTRY(PF_F32, float, K, 0, 0, 32, 8), PF_ENTRY(PF_F32, float, K, 0, 1, 64, 4),             \
-      PF_ENTRY(PF_F32, float, K, 0, 2, 32, 4), PF_ENTRY(PF_F32, float, K, 1, 0, 32, 8),         \
-      PF_ENTRY(PF_F32, float, K, 1, 1, 64, 4), PF_ENTRY(PF_F32, float, K, 1, 2, 32, 4)
+      PF_ENTRY(PF_F32, float, K, 0, 2, 16, 8), PF_ENTRY(PF_F32, float, K, 1, 0, 32, 8),         \
+      PF_ENTRY(PF_F32, float, K, 1, 1, 64, 4), PF_ENTRY(PF_F32, float, K, 1, 2, 16, 8)
 #elif PF_INST_DTYPE == 2
 #define PF_K_ENTRIES(K)                                                                          \
   PF_ENTRY(PF_BF16, bf16_t, K, 0, 0, 32, 8), PF_ENTRY(PF_BF16, bf16_t, K, 0, 1, 64, 4),         \
-      PF_ENTRY(PF_BF16, bf16_t, K, 0, 2, 32, 4), PF_ENTRY(PF_BF16, bf16_t, K, 1, 0, 32, 8),     \
-      PF_ENTRY(PF_BF16, bf16_t, K, 1, 1, 64, 4), PF_ENTRY(PF_BF16, bf16_t, K, 1, 2, 32, 4)
+      PF_ENTRY(PF_BF16, bf16_t, K, 0, 2, 16, 8), PF_ENTRY(PF_BF16, bf16_t, K, 1, 0, 32, 8),     \
+      PF_ENTRY(PF_BF16, bf16_t, K, 1, 1, 64, 4), PF_ENTRY(PF_BF16, bf16_t, K, 1, 2, 16, 8)
 #elif PF_INST_DTYPE == 0
 #define PF_K_ENTRIES(K)                                                                          \
   PF_ENTRY(PF_F64, double, K, 0, 0, 32, 4), PF_ENTRY(PF_F64, double, K, 0, 1, 16, 4),           \

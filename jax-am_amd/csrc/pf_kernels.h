@@ -24,16 +24,19 @@ struct WorkItem {
   int tile;
 };
 
-template <int K, class TS, bool ANISO, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void step_kernel(const StepDev a) {
+template <int K, class TS, bool ANISO, int TX, int TY, int H = split_of(K)>
+__global__ __launch_bounds__(TX* TY* H) __attribute__((amdgpu_waves_per_eu(2))) void step_kernel(const StepDev a) {
   using TC = typename Traits<TS>::TC;
-  __shared__ TC smem[2 * Tile<TS, K, TX, TY>::PLANE];
+  using G = Tile<TS, K, TX, TY, H>;
+  __shared__ __attribute__((aligned(16))) TC smem[2 * G::PLANE];
+  __shared__ __attribute__((aligned(16))) TS stage[G::STAGE];
+  __shared__ TC sring[G::SRING];
   const int n = a.tiles_x * a.tiles_y * a.tiles_z;
   const int t = xcd_swizzle(blockIdx.x, n);
   const int tx = t % a.tiles_x;
   const int ty = (t / a.tiles_x) % a.tiles_y;
   const int tz = t / (a.tiles_x * a.tiles_y);
-  march_tile<K, TS, ANISO, TX, TY>(a, tx, ty, tz, smem);
+  march_tile<K, TS, ANISO, TX, TY, H>(a, tx, ty, tz, smem, stage, sring);
 }
 
 template <class TS, bool ANISO, bool RHS = false>
@@ -58,15 +61,19 @@ __device__ __forceinline__ StepDev slot_view(const StepDev& d, const BatchSlot& 
 // Persistent batched step: a fixed grid walks the device-resident work list (slot,
 // tile) of one (dtype, K, anisotropy, tile shape) class.  Every workgroup exits after
 // the list is exhausted; slots with no steps left are skipped.
-template <int K, class TS, bool ANISO, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void batch_kernel(const StepDev* __restrict__ descs,
-                                                      const BatchSlot* __restrict__ slots,
-                                                      const WorkItem* __restrict__ work,
-                                                      const int* __restrict__ nwork) {
+template <int K, class TS, bool ANISO, int TX, int TY, int H = split_of(K)>
+__global__ __launch_bounds__(TX* TY* H) __attribute__((amdgpu_waves_per_eu(2))) void batch_kernel(const StepDev* __restrict__ descs,
+                                                          const BatchSlot* __restrict__ slots,
+                                                          const WorkItem* __restrict__ work_all,
+                                                          const int2* __restrict__ cinfo, int cls) {
   using TC = typename Traits<TS>::TC;
-  __shared__ TC smem[2 * Tile<TS, K, TX, TY>::PLANE];
-  const int nw = *nwork;
-  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+  using G = Tile<TS, K, TX, TY, H>;
+  __shared__ __attribute__((aligned(16))) TC smem[2 * G::PLANE];
+  __shared__ __attribute__((aligned(16))) TS stage[G::STAGE];
+  __shared__ TC sring[G::SRING];
+  const int2 ci = cinfo[cls];  // (offset, count) of this class's work list
+  const WorkItem* __restrict__ work = work_all + ci.x;
+  for (int w = blockIdx.x; w < ci.y; w += gridDim.x) {
     const WorkItem it = work[w];
     const BatchSlot s = slots[it.slot];
     if (s.steps_left <= 0 || !s.active) continue;
@@ -74,7 +81,7 @@ __global__ __launch_bounds__(TX* TY) void batch_kernel(const StepDev* __restrict
     const int tx = it.tile % a.tiles_x;
     const int ty = (it.tile / a.tiles_x) % a.tiles_y;
     const int tz = it.tile / (a.tiles_x * a.tiles_y);
-    march_tile<K, TS, ANISO, TX, TY>(a, tx, ty, tz, smem);
+    march_tile<K, TS, ANISO, TX, TY, H>(a, tx, ty, tz, smem, stage, sring);
     __syncthreads();  // LDS is reused by the next tile
   }
 }
@@ -83,10 +90,11 @@ __global__ __launch_bounds__(TX* TY) void batch_kernel(const StepDev* __restrict
 template <class TS, bool ANISO>
 __global__ __launch_bounds__(256) void batch_generic_kernel(const StepDev* __restrict__ descs,
                                                            const BatchSlot* __restrict__ slots,
-                                                           const WorkItem* __restrict__ work,
-                                                           const int* __restrict__ nwork) {
-  const int nw = *nwork;
-  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+                                                           const WorkItem* __restrict__ work_all,
+                                                           const int2* __restrict__ cinfo, int cls) {
+  const int2 ci = cinfo[cls];
+  const WorkItem* __restrict__ work = work_all + ci.x;
+  for (int w = blockIdx.x; w < ci.y; w += gridDim.x) {
     const WorkItem it = work[w];
     const BatchSlot s = slots[it.slot];
     if (s.steps_left <= 0 || !s.active) continue;
@@ -98,8 +106,8 @@ __global__ __launch_bounds__(256) void batch_generic_kernel(const StepDev* __res
 }
 
 using StepLaunch = hipError_t (*)(const StepDev&, hipStream_t);
-using BatchLaunch = hipError_t (*)(const StepDev*, const BatchSlot*, const WorkItem*, const int*, int grid,
-                                   hipStream_t);
+using BatchLaunch = hipError_t (*)(const StepDev*, const BatchSlot*, const WorkItem*, const int2*, int cls,
+                                   int grid, hipStream_t);
 
 struct StepEntry {
   int dtype, k, aniso, tile, tx, ty;
@@ -111,14 +119,14 @@ struct StepEntry {
 template <int K, class TS, bool ANISO, int TX, int TY>
 hipError_t launch_step(const StepDev& a, hipStream_t s) {
   const int n = a.tiles_x * a.tiles_y * a.tiles_z;
-  step_kernel<K, TS, ANISO, TX, TY><<<dim3(n), dim3(TX * TY), 0, s>>>(a);
+  step_kernel<K, TS, ANISO, TX, TY><<<dim3(n), dim3(TX * TY * split_of(K)), 0, s>>>(a);
   return hipGetLastError();
 }
 
 template <int K, class TS, bool ANISO, int TX, int TY>
-hipError_t launch_batch(const StepDev* d, const BatchSlot* sl, const WorkItem* w, const int* nw, int grid,
+hipError_t launch_batch(const StepDev* d, const BatchSlot* sl, const WorkItem* w, const int2* ci, int cls, int grid,
                         hipStream_t s) {
-  batch_kernel<K, TS, ANISO, TX, TY><<<dim3(grid), dim3(TX * TY), 0, s>>>(d, sl, w, nw);
+  batch_kernel<K, TS, ANISO, TX, TY><<<dim3(grid), dim3(TX * TY * split_of(K)), 0, s>>>(d, sl, w, ci, cls);
   return hipGetLastError();
 }
 
@@ -130,15 +138,15 @@ hipError_t launch_generic(const StepDev& a, hipStream_t s) {
 }
 
 template <class TS, bool ANISO>
-hipError_t launch_batch_generic(const StepDev* d, const BatchSlot* sl, const WorkItem* w, const int* nw, int grid,
-                                hipStream_t s) {
-  batch_generic_kernel<TS, ANISO><<<dim3(grid), dim3(256), 0, s>>>(d, sl, w, nw);
+hipError_t launch_batch_generic(const StepDev* d, const BatchSlot* sl, const WorkItem* w, const int2* ci, int cls,
+                                int grid, hipStream_t s) {
+  batch_generic_kernel<TS, ANISO><<<dim3(grid), dim3(256), 0, s>>>(d, sl, w, ci, cls);
   return hipGetLastError();
 }
 
 #define PF_ENTRY(DT, TS, K, A, T, TX, TY)                                                             \
   {DT, K, A, T, TX, TY, &launch_step<K, TS, (A != 0), TX, TY>, &launch_batch<K, TS, (A != 0), TX, TY>, \
-   Tile<TS, K, TX, TY>::LDS_BYTES}
+   Tile<TS, K, TX, TY, split_of(K)>::LDS_BYTES}
 
 // instantiation tables, one per compiled part of pf_inst.hip (see Makefile):
 // parts 0-3 F32, 4-7 BF16, 8-11 F64, each over a K range

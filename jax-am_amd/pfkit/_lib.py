@@ -11,7 +11,7 @@ import os
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpfhip.so")
+LIB_PATH = os.environ.get("PFKIT_LIB") or os.path.join(_HERE, "libpfhip.so")
 
 PF_F64, PF_F32, PF_BF16 = 0, 1, 2
 PF_T_BUFFER, PF_T_ROSENTHAL, PF_T_GAUSSIAN, PF_T_QUENCH = 0, 1, 2, 3
@@ -65,6 +65,7 @@ SIGNATURES = {
     "pf_temperature": (ctypes.c_int, [_VP, _I32, ctypes.POINTER(ParamsT), _D, _VP]),
     "pf_init_onehot": (ctypes.c_int, [_VP, _I32, _VP, _I64, _I32, _VP]),
     "pf_argmax": (ctypes.c_int, [_VP, _I32, _I64, _I32, _VP, _VP]),
+    "pf_voronoi": (ctypes.c_int, [_VP, _I32, _VP, ctypes.POINTER(ParamsT), _VP, _VP]),
     "pf_reduce_workspace_bytes": (_SZ, [_I64, _I32]),
     "pf_reduce": (ctypes.c_int, [_VP, _I32, _VP, _VP, ctypes.POINTER(ParamsT), _D, _VP, _VP, _VP, _SZ, _VP]),
     "pf_batch_create": (_VP, [_I32, _I32, _VP]),

@@ -199,7 +199,11 @@ class PFSolver:
     def ini_cond(self):
         """allen_cahn.py:145-153: one-hot eta[c, cell_ori_inds[c]] = 1."""
         eta = torch.empty((self.N, self.K), dtype=self.dtype, device=self.device)
-        ori = torch.as_tensor(np.asarray(self.polycrystal.cell_ori_inds, dtype=np.int32), device=self.device)
+        ori = self.polycrystal.cell_ori_inds
+        if isinstance(ori, torch.Tensor):
+            ori = ori.to(device=self.device, dtype=torch.int32)
+        else:
+            ori = torch.as_tensor(np.asarray(ori, dtype=np.int32), device=self.device)
         lib = _lib.load()
         _lib.check(lib.pf_init_onehot(eta.data_ptr(), self.state_rhs.pf_dtype, ori.data_ptr(), self.N, self.K,
                                       _lib.stream_handle(self.device)), "pf_init_onehot")

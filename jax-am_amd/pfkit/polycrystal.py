@@ -93,6 +93,41 @@ class Polycrystal:
         return self.Nx * self.Ny * self.Nz
 
 
+class DevicePolycrystal:
+    """The same polycrystal as ``Polycrystal`` (same seeds, grain orientations and quaternions for
+    the same arguments), with the grain map computed on the GPU (pf_voronoi) and kept there:
+    ``cell_ori_inds`` is a device int32 tensor.  For request-time initial states."""
+
+    def __init__(self, Nx, Ny, Nz, h_xyz, num_grains, num_oris, seed=1, device=None):
+        import torch
+
+        from . import _lib
+        from .config import make_params
+        self.Nx, self.Ny, self.Nz = int(Nx), int(Ny), int(Nz)
+        self.mesh_h_xyz = tuple(float(h) for h in h_xyz)
+        self.num_grains, self.num_oris = int(num_grains), int(num_oris)
+        L = np.array([self.Nx * self.mesh_h_xyz[0], self.Ny * self.mesh_h_xyz[1], self.Nz * self.mesh_h_xyz[2]])
+        self.seeds = np.random.default_rng(seed).random((self.num_grains, 3)) * L
+        grain_oris_inds = np.random.RandomState(seed).randint(self.num_oris, size=self.num_grains)
+        self.quat = random_quaternions(self.num_oris, seed)
+        self.unique_grain_directions = grain_directions(self.quat)
+        self.unique_oris_rgb = ipf_colors(self.quat)
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        sd = torch.as_tensor(self.seeds, dtype=torch.float64, device=dev)
+        go = torch.as_tensor(grain_oris_inds.astype(np.int32), device=dev)
+        out = torch.empty(self.n_cells, dtype=torch.int32, device=dev)
+        p = make_params(dict(Nx=self.Nx, Ny=self.Ny, Nz=self.Nz, num_oris=self.num_oris, m_g=1.0, kappa_g=1.0,
+                             L0=1.0, Qg=1.0, gas_const=1.0, T_liquidus=1.0), self.mesh_h_xyz)
+        lib = _lib.load()
+        _lib.check(lib.pf_voronoi(sd.data_ptr(), self.num_grains, go.data_ptr(), p, out.data_ptr(),
+                                  _lib.stream_handle(dev)), "pf_voronoi")
+        self.cell_ori_inds = out
+
+    @property
+    def n_cells(self):
+        return self.Nx * self.Ny * self.Nz
+
+
 class Field(Polycrystal):
     """Constructor in the reference's form ``Field(pf_args, ori2=None)`` (utils.py:19-26).
 

@@ -51,17 +51,30 @@ class Workload:
     def K(self):
         return int(self.pf_args["num_oris"])
 
+    def instance(self, **kw):
+        """The batch.Instance of this workload (one-hot start)."""
+        from .batch import Instance
+        return Instance(pf_args=self.pf_args, h_xyz=self.polycrystal.mesh_h_xyz,
+                        grain_dirs=self.polycrystal.unique_grain_directions, laser=self.laser, dtype=self.dtype,
+                        steps=self.steps, dt=self.dt, cell_ori_inds=self.polycrystal.cell_ori_inds, tag=self.name,
+                        **kw)
+
     def ts(self, steps=None):
         n = self.steps if steps is None else steps
         return np.arange(n + 1) * self.dt   # np.arange time grid (simple/example.py:52)
 
 
-def make(name, Nx, Ny, Nz, K, grains, steps, dtype, anisotropy, seed=1, vel=None):
+def make(name, Nx, Ny, Nz, K, grains, steps, dtype, anisotropy, seed=1, vel=None, device_grains=False):
+    """A workload; device_grains=True builds the grain map on the GPU (DevicePolycrystal)."""
     h = (1.0, 1.0, 1.0)
     a = with_defaults(dict(PHYSICS, Nx=Nx, Ny=Ny, Nz=Nz, num_oris=K, num_grains=grains,
                            anisotropy=anisotropy, domain_x=Nx * h[0], domain_y=Ny * h[1], domain_z=Nz * h[2]))
     dt, dt_max, c = stable_dt(a, h)
-    pc = Polycrystal(Nx, Ny, Nz, h, grains, K, seed=seed)
+    if device_grains:
+        from .polycrystal import DevicePolycrystal
+        pc = DevicePolycrystal(Nx, Ny, Nz, h, grains, K, seed=seed)
+    else:
+        pc = Polycrystal(Nx, Ny, Nz, h, grains, K, seed=seed)
     Lx, Ly, Lz = Nx * h[0], Ny * h[1], Nz * h[2]
     if vel is None:
         vel = 0.6 * Lx / (steps * dt)   # crosses 0.6 Lx during the run, like simple/example.py:30
@@ -89,7 +102,7 @@ def _loguniform_dims(rng, lo, hi):
     return tuple(int(round(l * (h / l) ** u)) for l, h in zip(lo, hi))
 
 
-def c3_instances(n=64, seed=3):
+def c3_instances(n=64, seed=3, device_grains=True):
     rng = np.random.default_rng(seed)
     out = []
     for j in range(n):
@@ -98,15 +111,16 @@ def c3_instances(n=64, seed=3):
         vel = float(rng.uniform(0.5, 2.0))
         steps = int(rng.integers(200, 1001))
         grains = max(8, (Nx * Ny * Nz) // 8000)
-        out.append(make(f"c3_{j}", Nx, Ny, Nz, K, grains, steps, "float32", 0.15, seed=seed * 1000 + j, vel=vel))
+        out.append(make(f"c3_{j}", Nx, Ny, Nz, K, grains, steps, "float32", 0.15, seed=seed * 1000 + j, vel=vel,
+                        device_grains=device_grains))
     return out
 
 
-def c4_request(rng, j):
+def c4_request(rng, j, device_grains=True):
     Nx, Ny, Nz = _loguniform_dims(rng, (32, 32, 32), (256, 256, 64))
     K = int(rng.integers(10, 21))
     steps = int(rng.integers(100, 2001))
     dtype = "float32" if rng.random() < 0.5 else "bfloat16"
     grains = max(8, (Nx * Ny * Nz) // 8000)
     return make(f"c4_{j}", Nx, Ny, Nz, K, grains, steps, dtype, 0.15, seed=100000 + j,
-                vel=float(rng.uniform(0.5, 2.0)))
+                vel=float(rng.uniform(0.5, 2.0)), device_grains=device_grains)

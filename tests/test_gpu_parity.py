@@ -110,17 +110,40 @@ def test_tile_shapes_and_generic_kernel_agree_bitwise(cuda):
     a = args(Nx, Ny, Nz, K, anisotropy=0.15, force_liquid=True)
     pc = poly(Nx, Ny, Nz, K)
     L = gaussian(Nx, Ny, Nz)
-    eta = random_eta(Nx * Ny * Nz, K, seed=5)
-    for dtype in (torch.float64, torch.float32, torch.bfloat16):
+    dense = random_eta(Nx * Ny * Nz, K, seed=5)
+    # sparse: one-hot grains with a diffuse band and a few -0.0 entries (zero-channel skips,
+    # zero-gradient anisotropy skips and cold/hot temperature paths all occur)
+    sparse = O.ini_cond(pc.cell_ori_inds, K)
+    band = (np.arange(Nx * Ny * Nz) % 7) == 0
+    sparse[band] = 0.8 * sparse[band] + 0.2 * dense[band]
+    sparse[3, 1] = -0.0
+    for eta in (dense, sparse):
+        for dtype in (torch.float64, torch.float32, torch.bfloat16):
+            s = _solver(a, pc, dtype, cuda)
+            lib = _lib.load()
+            nt = lib.pf_tile_count(_lib.TORCH_TO_PF[dtype], K)
+            assert nt >= 2
+            base = _gpu_step(s, eta, L, 0.1, 0.12, dtype, tile=-2)  # generic kernel
+            for tile in range(nt):
+                for zc in (0, 1, 3, 7, Nz):
+                    got = _gpu_step(s, eta, L, 0.1, 0.12, dtype, tile=tile, zchunk=zc)
+                    assert torch.equal(got, base), (dtype, tile, zc)
+
+
+def test_sparse_one_hot_trajectory_f32_and_f64(cuda):
+    """From the one-hot state most channels are exactly zero: exercises the skip paths."""
+    Nx, Ny, Nz, K = 40, 24, 12, 10
+    a = args(Nx, Ny, Nz, K, anisotropy=0.15, force_liquid=True)
+    pc = poly(Nx, Ny, Nz, K, grains=30)
+    L = gaussian(Nx, Ny, Nz, vel=4.0)
+    eta0 = O.ini_cond(pc.cell_ori_inds, K)
+    ts = np.arange(0.0, 40 * 0.025 + 1e-10, 0.025)
+    ref = O.run(eta0, ts, a, pc.mesh_h_xyz, pc.unique_grain_directions, laser=L.as_dict(),
+                cent=O.centroids(Nz, Ny, Nx, pc.mesh_h_xyz))
+    for dtype, tol in ((torch.float64, 1e-11), (torch.float32, 2e-5)):
         s = _solver(a, pc, dtype, cuda)
-        lib = _lib.load()
-        nt = lib.pf_tile_count(_lib.TORCH_TO_PF[dtype], K)
-        assert nt >= 2
-        base = _gpu_step(s, eta, L, 0.1, 0.12, dtype, tile=-2)  # generic kernel
-        for tile in range(nt):
-            for zc in (0, 1, 3, 7, Nz):
-                got = _gpu_step(s, eta, L, 0.1, 0.12, dtype, tile=tile, zchunk=zc)
-                assert torch.equal(got, base), (dtype, tile, zc)
+        got = s.run(s.ini_cond(), ts, L).double().cpu().numpy()
+        assert np.max(np.abs(got - ref)) <= tol * np.max(np.abs(ref)), dtype
 
 
 def test_t_buffer_matches_laser_and_oracle(cuda):
