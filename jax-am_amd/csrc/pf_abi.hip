@@ -76,6 +76,10 @@ int make_stepdev(const pf_params_t* p, int dtype, const void* T_dev, const void*
   a.kappa_ad = p->kappa_g * p->ad_hoc;
   a.gamma4 = 4.0 * p->gamma;
   a.B = 1.0 - 4.0 * p->gamma;
+  a.ihx2f = (float)a.ihx2;
+  a.ihy2f = (float)a.ihy2;
+  a.ihz2f = (float)a.ihz2;
+  a.Bf = (float)a.B;
   a.L0 = p->L0;
   a.Qg = p->Qg;
   a.gas_const = p->gas_const;
@@ -495,10 +499,7 @@ int pf_step(const void* eta_in_dev, void* eta_out_dev, int32_t dtype, const void
   StepDev a;
   int rc = make_stepdev(p, dtype, T_dev, axes_dev, zchunk, a);
   if (rc) return rc;
-  a.in = eta_in_dev;
-  a.out = eta_out_dev;
-  a.t_prev = t_prev;
-  a.hstep = t_crt - t_prev;  // allen_cahn.py:112
+  const StepDyn d = make_dyn(a, eta_in_dev, eta_out_dev, t_prev, t_crt - t_prev);  // allen_cahn.py:112
   const bool aniso = p->anisotropy > 0.0;
   if (tile < -2) return fail(PF_E_ARG, "tile must be >= -2");
   const StepEntry* e = tile == -2 ? nullptr : find_entry(dtype, p->num_oris, aniso, tile);
@@ -508,12 +509,12 @@ int pf_step(const void* eta_in_dev, void* eta_out_dev, int32_t dtype, const void
     set_tiles(a, e->tx, e->ty, zchunk);
     const long long nt = (long long)a.tiles_x * a.tiles_y * a.tiles_z;
     if (nt >= (1LL << 31)) return fail(PF_E_ARG, "too many tiles");
-    err = e->step(a, (hipStream_t)stream);
+    err = e->step(a, d, (hipStream_t)stream);
   } else {
     if (tile > 0 && entry_count(dtype, p->num_oris, aniso) > 0) return fail(PF_E_ARG, "no such tile shape");
     const long long n = (long long)p->nz * p->ny * p->nx;
     if (!grid_ok(n, 1)) return fail(PF_E_ARG, "grid too large for the generic kernel");
-    err = generic_step(dtype, aniso)(a, (hipStream_t)stream);
+    err = generic_step(dtype, aniso)(a, d, (hipStream_t)stream);
   }
   if (err != hipSuccess) return hip_fail(err, "pf_step launch");
   return PF_OK;
@@ -526,13 +527,10 @@ int pf_rhs(const void* eta_in_dev, void* rhs_out_dev, int32_t dtype, const void*
   StepDev a;
   int rc = make_stepdev(p, dtype, T_dev, axes_dev, 0, a);
   if (rc) return rc;
-  a.in = eta_in_dev;
-  a.out = rhs_out_dev;
-  a.t_prev = t;
-  a.hstep = 1.0;  // c1, c2 then carry L only: rhs = -L (grad + laplace)
+  const StepDyn d = make_dyn(a, eta_in_dev, rhs_out_dev, t, 1.0);  // c1, c2 carry L only: rhs = -L (grad + laplace)
   const long long n = (long long)p->nz * p->ny * p->nx;
   if (!grid_ok(n, 1)) return fail(PF_E_ARG, "grid too large");
-  hipError_t err = generic_rhs(dtype, p->anisotropy > 0.0)(a, (hipStream_t)stream);
+  hipError_t err = generic_rhs(dtype, p->anisotropy > 0.0)(a, d, (hipStream_t)stream);
   return err == hipSuccess ? PF_OK : hip_fail(err, "pf_rhs launch");
 }
 

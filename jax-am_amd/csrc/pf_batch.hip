@@ -1,8 +1,9 @@
 // Batched engine of libpfhip.so: many independent phase-field instances advanced
 // together.  Design:
-//   - Slot table (StepDev descriptors + BatchSlot step counters) lives in device
-//     memory; the device advances each slot's step index and ping-pong parity, so a
-//     captured graph stays valid while instances come and go.
+//   - Slot table (StepDev descriptors, BatchSlot step counters and each slot's StepDyn
+//     step arguments) lives in device memory; the device advances each slot's step
+//     index and ping-pong parity and rebuilds its StepDyn, so a captured graph stays
+//     valid while instances come and go.
 //   - Instances are grouped into classes (dtype, K, anisotropy, tile shape).  Each
 //     class has a device-resident work list of (slot, tile) items walked by one
 //     persistent launch per step with a fixed grid.
@@ -23,7 +24,8 @@
 
 namespace pf {
 
-__global__ __launch_bounds__(256) void advance_kernel(BatchSlot* __restrict__ slots, int n) {
+__global__ __launch_bounds__(64) void advance_kernel(const StepDev* __restrict__ descs, BatchSlot* __restrict__ slots,
+                                                    StepDyn* __restrict__ dyns, int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   BatchSlot b = slots[s];
@@ -32,6 +34,7 @@ __global__ __launch_bounds__(256) void advance_kernel(BatchSlot* __restrict__ sl
     b.parity ^= 1;
     b.n += 1;
     slots[s] = b;
+    dyns[s] = slot_dyn(descs[s], b);
   }
 }
 
@@ -58,6 +61,7 @@ struct pf_batch {
   static constexpr int kMaxClasses = 64;
   hipStream_t stream = nullptr;
   StepDev* d_desc = nullptr;
+  StepDyn* d_dyn = nullptr;
   BatchSlot* d_slots = nullptr;
   WorkItem* d_work = nullptr;
   int2* d_info = nullptr;  // per class (work-list offset, count)
@@ -117,10 +121,10 @@ int launch_step_all(pf_batch* b) {
   for (size_t c = 0; c < b->classes.size(); ++c) {
     const BatchClass& k = b->classes[c];
     if (k.count == 0) continue;
-    hipError_t e = k.fn(b->d_desc, b->d_slots, b->d_work, b->d_info, (int)c, k.grid, b->stream);
+    hipError_t e = k.fn(b->d_desc, b->d_dyn, b->d_slots, b->d_work, b->d_info, (int)c, k.grid, b->stream);
     if (e != hipSuccess) return hip_fail(e, "pf_batch step launch");
   }
-  advance_kernel<<<(b->max_inst + 255) / 256, 256, 0, b->stream>>>(b->d_slots, b->max_inst);
+  advance_kernel<<<(b->max_inst + 63) / 64, 64, 0, b->stream>>>(b->d_desc, b->d_slots, b->d_dyn, b->max_inst);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? PF_OK : hip_fail(e, "pf_batch advance launch");
 }
@@ -189,6 +193,7 @@ pf_batch_t* pf_batch_create(int32_t max_instances, int32_t max_tiles, void* stre
   b->stream = (hipStream_t)stream;
   b->slots.resize(max_instances);
   hipError_t e = hipMalloc((void**)&b->d_desc, sizeof(StepDev) * max_instances);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_dyn, sizeof(StepDyn) * max_instances);
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_slots, sizeof(BatchSlot) * max_instances);
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_work, sizeof(WorkItem) * max_tiles);
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_info, sizeof(int2) * pf_batch::kMaxClasses);
@@ -214,6 +219,7 @@ void pf_batch_destroy(pf_batch_t* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graphs(b);
   if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_dyn) (void)hipFree(b->d_dyn);
   if (b->d_slots) (void)hipFree(b->d_slots);
   if (b->d_work) (void)hipFree(b->d_work);
   if (b->d_info) (void)hipFree(b->d_info);
@@ -282,10 +288,12 @@ int pf_batch_add(pf_batch_t* b, const pf_batch_inst_t* inst) {
   st.dt = inst->dt;
   st.eta[0] = inst->eta_dev[0];
   st.eta[1] = inst->eta_dev[1];
+  const StepDyn dy = slot_dyn(a, st);
   hipError_t he = hipMemcpyAsync(b->d_desc + slot, &a, sizeof(StepDev), hipMemcpyHostToDevice, b->stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(b->d_dyn + slot, &dy, sizeof(StepDyn), hipMemcpyHostToDevice, b->stream);
   if (he == hipSuccess)
     he = hipMemcpyAsync(b->d_slots + slot, &st, sizeof(BatchSlot), hipMemcpyHostToDevice, b->stream);
-  if (he == hipSuccess) he = hipStreamSynchronize(b->stream);  // a and st are locals
+  if (he == hipSuccess) he = hipStreamSynchronize(b->stream);  // a, dy and st are locals
   if (he != hipSuccess) return hip_fail(he, "pf_batch_add upload");
   b->slots[slot].used = true;
   b->slots[slot].cls = cls;

@@ -13,16 +13,17 @@
 // a = 1 + eps*(q^2 + (1-q)^2), q = max_k (d_k.e)^2 / |e|^2 clamped to 1 (q = 1 when
 // |e| = 0, the reference's NaN -> angle 0 rule).
 //
-// Per-cell quantities (temperature, liquid indicator zeta, Arrhenius mobility,
-// liquid reset) are evaluated in FP64 for every storage dtype so that the
-// liquid/solid decision matches an FP64 run; per-orientation arithmetic is done
-// in the compute type TC (float for F32/BF16 storage, double for F64).
-// All device code is built with -ffp-contract=off: every fused multiply-add
-// below is written out, so each kernel that shares these functions produces
-// bitwise the same result for the same cell.
+// Per-cell quantities (temperature, liquid indicator zeta, Arrhenius mobility, liquid
+// reset) are evaluated in FP64, except that with FP32 arithmetic a cell whose mobility is
+// negligible (make_cold) takes an FP32 mobility; per-orientation
+// arithmetic is done in the compute type TC (float for F32/BF16 storage, double for F64).
+// All device code is built with -ffp-contract=off: every fused multiply-add below is
+// written out, so each kernel that shares these functions produces bitwise the same
+// result for the same cell.
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 #include "pfhip.h"
@@ -56,16 +57,12 @@ template <class T, int N> struct alignas(sizeof(T) * N) Vec {
   T v[N];
 };
 
-// non-temporal vector store of a 2/4/8/16-byte value
+// raw vector type of a 2/4/8/16-byte piece
 template <int BYTES> struct RawT;
 template <> struct RawT<16> { typedef unsigned int type __attribute__((ext_vector_type(4))); };
 template <> struct RawT<8> { typedef unsigned int type __attribute__((ext_vector_type(2))); };
 template <> struct RawT<4> { typedef unsigned int type; };
 template <> struct RawT<2> { typedef unsigned short type; };
-template <class V> __device__ __forceinline__ void store_nt(V* p, const V& v) {
-  using R = typename RawT<sizeof(V)>::type;
-  __builtin_nontemporal_store(__builtin_bit_cast(R, v), (R*)p);
-}
 
 __device__ __forceinline__ float to_tc(float x) { return x; }
 __device__ __forceinline__ double to_tc(double x) { return x; }
@@ -87,22 +84,20 @@ __device__ __forceinline__ double fmac(double a, double b, double c) { return __
 __device__ __forceinline__ float rcp_approx(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // ------------------------------------------------------------------ kernel arguments
-// Everything a tile march needs about one instance.  Built on the host from
-// pf_params_t (pf_host.h) and passed by value (single step) or read from device
-// memory (batched engine).
+// Static description of one instance: grid, physics and temperature model.  Built on the
+// host from pf_params_t (make_stepdev) and passed as a kernel argument (single step) or
+// kept in device memory (batched engine, reduction).
 struct StepDev {
-  const void* in;
-  void* out;
   const void* T;       // PF_T_BUFFER temperature
   const void* axes;    // [3][K][3] in TC
   int nz, ny, nx, k;
   int tiles_x, tiles_y, tiles_z, zc;
-  int tmodel, t_f64, force_liquid;
+  int tmodel, t_f64, force_liquid, pad0;
+  float ihx2f, ihy2f, ihz2f, Bf;          // FP32 copies of ihx2, ihy2, ihz2, B
   double hx, hy, hz;
   double ihx2, ihy2, ihz2;
   double m_over_ad, kappa_ad, gamma4, B;  // m_g/ad_hoc, kappa_g*ad_hoc, 4*gamma, 1-4*gamma
   double L0, Qg, gas_const, T_l, T_clip, theta, zeta_window, eps, liquid_T;
-  double t_prev, hstep;
   double stiff;  // m_g/ad_hoc (4 + 4|gamma|) + 4 kappa_g ad_hoc (1 + eps) sum 1/h^2 (cold-path threshold)
   // laser
   double T_amb, p0x, p0y, p0z, dirx, diry, dirz, vel;
@@ -111,10 +106,79 @@ struct StepDev {
   double T_ref, grad_T, cool_rate;   // quench
 };
 
-// ------------------------------------------------------------------ per-cell physics (FP64)
-// Temperature at the centroid of cell (ix, iy, iz) at time t; cell index c for the buffer mode.
-__device__ __forceinline__ double cell_temperature(const StepDev& a, int ix, int iy, int iz,
-                                                   long long c, double t) {
+// FP32 constants of one step: the temperature model at t and the cold-path mobility.
+struct ColdPath {
+  float px, py, pz;           // source position at t
+  float hx, hy, hz;
+  float hs, T_lim;            // |hstep * stiff|; cold-path cells have T < T_lim and L(T) * hs < 1e-6
+  float L0, Qg_R;             // L0, Qg / gas_const (Arrhenius)
+  float hm, hk, eps1, eps2;   // hstep*m_g/ad_hoc, hstep*kappa_g*ad_hoc, 1+eps, 2*eps
+  float gamma4;
+  float T_amb, dT, gxy, gz, q, pe, dxr, dyr, dzr, T_cap, T_ref, grad_T, cool_t;
+};
+
+// One step of one instance: state buffers, time and the FP32 step constants.
+struct StepDyn {
+  const void* in;
+  void* out;
+  double t_prev, hstep;
+  ColdPath cold;
+};
+
+// Cold path: a cell whose FP32 mobility satisfies L(T) * |hstep * stiff| < 1e-6 changes by
+// less than 1e-6 * |eta| per step, so the ~1e-6 relative error of an FP32 mobility moves
+// it by under 1e-12.  Cold cells must also lie at least 10 K below T_liquidus, the
+// liquid-reset threshold and T_clip, so zeta = 1 and no reset are the exact FP64 decisions.
+// No transcendental is evaluated here, so host and device build bitwise the same constants.
+__host__ __device__ inline ColdPath make_cold(const StepDev& a, double t, double hstep) {
+  ColdPath k;
+  const double vt = a.vel * t;
+  k.px = (float)(a.p0x + vt * a.dirx);
+  k.py = (float)(a.p0y + vt * a.diry);
+  k.pz = (float)(a.p0z + vt * a.dirz);
+  k.hx = (float)a.hx;
+  k.hy = (float)a.hy;
+  k.hz = (float)a.hz;
+  k.hs = (float)fabs(hstep * a.stiff);
+  k.T_lim = (float)(fmin(a.T_l, fmin(a.liquid_T, a.T_clip)) - 10.0);
+  k.L0 = (float)a.L0;
+  k.Qg_R = (float)(a.Qg / a.gas_const);
+  k.hm = (float)(hstep * a.m_over_ad);
+  k.hk = (float)(hstep * a.kappa_ad);
+  k.eps1 = (float)(1.0 + a.eps);
+  k.eps2 = (float)(2.0 * a.eps);
+  k.gamma4 = (float)a.gamma4;
+  k.T_amb = (float)a.T_amb;
+  k.dT = (float)a.dT_peak;
+  k.gxy = (float)a.g_sxy;
+  k.gz = (float)a.g_sz;
+  k.q = (float)a.q_coef;
+  k.pe = (float)a.pe;
+  k.dxr = (float)a.dirx;
+  k.dyr = (float)a.diry;
+  k.dzr = (float)a.dirz;
+  k.T_cap = (float)a.T_cap;
+  k.T_ref = (float)a.T_ref;
+  k.grad_T = (float)a.grad_T;
+  k.cool_t = (float)(a.cool_rate * t);
+  return k;
+}
+
+__host__ __device__ inline StepDyn make_dyn(const StepDev& a, const void* in, void* out, double t_prev,
+                                            double hstep) {
+  StepDyn d;
+  d.in = in;
+  d.out = out;
+  d.t_prev = t_prev;
+  d.hstep = hstep;
+  d.cold = make_cold(a, t_prev, hstep);
+  return d;
+}
+
+// ------------------------------------------------------------------ per-cell physics
+// Temperature at the centroid of cell (ix, iy, iz) at time t (FP64); cell index c for the buffer mode.
+__device__ __forceinline__ double cell_temperature(const StepDev& a, int ix, int iy, int iz, long long c,
+                                                   double t) {
   if (a.tmodel == PF_T_BUFFER) {
     return a.t_f64 ? ((const double*)a.T)[c] : (double)((const float*)a.T)[c];
   }
@@ -141,71 +205,9 @@ __device__ __forceinline__ double cell_temperature(const StepDev& a, int ix, int
   return a.T_amb + a.dT_peak * exp(-((X * X + Y * Y) * a.g_sxy + Z * Z * a.g_sz));
 }
 
-template <class TC> struct CellCoef {
-  TC c1, c2, A, w1, w2;  // w1 = c2*(1+eps), w2 = 2*eps*c2 (anisotropic factor folded into c2)
-  bool liquid;
-};
-
-// ---- FP32 temperature for the cold-cell path
-// Launch-uniform FP32 constants of the temperature model at time t and the FP64/FP32
-// path threshold.  Built once per thread (a few FP64 operations).
-struct ColdPath {
-  float px, py, pz;        // source position at t
-  float hx, hy, hz;
-  float T_floor;           // cells with T < T_floor take the FP32 mobility path
-  float L0, Qg_R;          // L0, Qg / gas_const (Arrhenius)
-  float hm, hk, eps1, eps2;   // hstep*m_g/ad_hoc, hstep*kappa_g*ad_hoc, 1+eps, 2*eps
-  float gamma4;
-  float T_amb, dT, gxy, gz, q, pe, dxr, dyr, dzr, T_cap, T_ref, grad_T, cool_t;
-};
-
-// T_floor: the temperature below which the mobility is so small that its FP32 evaluation
-// (relative error ~1e-6) changes an update by less than 1e-12:  L(T_floor) * hstep * stiff
-// = 1e-6, stiff = m_g/ad_hoc (4 + 4 gamma) + 4 kappa_g ad_hoc (1 + eps) sum 1/h^2.  Kept at
-// least 10 K below T_liquidus, the liquid-reset threshold and T_clip, so the cold path's
-// zeta = 1 and no-reset are the exact FP64 decisions.
-__device__ __forceinline__ ColdPath make_cold(const StepDev& a, double t) {
-  ColdPath k;
-  const double vt = a.vel * t;
-  k.px = (float)(a.p0x + vt * a.dirx);
-  k.py = (float)(a.p0y + vt * a.diry);
-  k.pz = (float)(a.p0z + vt * a.dirz);
-  k.hx = (float)a.hx;
-  k.hy = (float)a.hy;
-  k.hz = (float)a.hz;
-  double tf = -INFINITY;
-  const double sc = a.hstep * a.stiff;
-  if (sc > 0.0 && a.L0 > 0.0 && a.Qg > 0.0 && a.gas_const > 0.0) {
-    const double Lfl = 1e-6 / sc;
-    tf = Lfl < a.L0 ? -a.Qg / (a.gas_const * log(Lfl / a.L0)) : INFINITY;
-    tf = fmin(tf, fmin(a.T_l, fmin(a.liquid_T, a.T_clip)) - 10.0);
-  }
-  k.T_floor = (float)tf;
-  k.L0 = (float)a.L0;
-  k.Qg_R = (float)(a.Qg / a.gas_const);
-  k.hm = (float)(a.hstep * a.m_over_ad);
-  k.hk = (float)(a.hstep * a.kappa_ad);
-  k.eps1 = (float)(1.0 + a.eps);
-  k.eps2 = (float)(2.0 * a.eps);
-  k.gamma4 = (float)a.gamma4;
-  k.T_amb = (float)a.T_amb;
-  k.dT = (float)a.dT_peak;
-  k.gxy = (float)a.g_sxy;
-  k.gz = (float)a.g_sz;
-  k.q = (float)a.q_coef;
-  k.pe = (float)a.pe;
-  k.dxr = (float)a.dirx;
-  k.dyr = (float)a.diry;
-  k.dzr = (float)a.dirz;
-  k.T_cap = (float)a.T_cap;
-  k.T_ref = (float)a.T_ref;
-  k.grad_T = (float)a.grad_T;
-  k.cool_t = (float)(a.cool_rate * t);
-  return k;
-}
-
-__device__ __forceinline__ float cell_temperature_f32(const StepDev& a, const ColdPath& k, int ix, int iy,
-                                                      int iz, long long c) {
+// The same temperature in FP32 (cold-path test and value).
+__device__ __forceinline__ float cell_temperature_f32(const StepDev& a, const ColdPath& k, int ix, int iy, int iz,
+                                                      long long c) {
   if (a.tmodel == PF_T_BUFFER) return a.t_f64 ? (float)((const double*)a.T)[c] : ((const float*)a.T)[c];
   const float cz = ((float)iz + 0.5f) * k.hz;
   if (a.tmodel == PF_T_QUENCH) return k.T_ref + k.grad_T * cz - k.cool_t;
@@ -222,9 +224,14 @@ __device__ __forceinline__ float cell_temperature_f32(const StepDev& a, const Co
   return k.T_amb + k.dT * expf(-((X * X + Y * Y) * k.gxy + Z * Z * k.gz));
 }
 
+template <class TC> struct CellCoef {
+  TC c1, c2, A, w1, w2;  // w1 = c2*(1+eps), w2 = 2*eps*c2 (anisotropic factor folded into c2)
+  bool liquid;
+};
+
 // Per-cell coefficients of the update (allen_cahn.py:55-58,83-84,124-126), FP64 evaluation.
 template <class TC>
-__device__ __forceinline__ CellCoef<TC> cell_coef64(const StepDev& a, double T, TC S) {
+__device__ __forceinline__ CellCoef<TC> cell_coef64(const StepDev& a, double T, TC S, double hstep) {
   CellCoef<TC> r;
   const double Tc = T > a.T_clip ? a.T_clip : T;                 // :56
   // :57  zeta = 0.5*(1 - tanh(theta*(Tc/T_l - 1))).  In FP64, tanh(x) rounds to exactly +-1
@@ -241,7 +248,7 @@ __device__ __forceinline__ CellCoef<TC> cell_coef64(const StepDev& a, double T, 
   }
   const double omz = 1.0 - zeta;
   const double Lg = a.L0 * exp(-a.Qg / (Tc * a.gas_const));      // :83
-  const double hL = a.hstep * Lg;
+  const double hL = hstep * Lg;
   const double c1 = hL * a.m_over_ad;
   const double c2 = hL * a.kappa_ad;
   r.c1 = (TC)c1;
@@ -254,21 +261,21 @@ __device__ __forceinline__ CellCoef<TC> cell_coef64(const StepDev& a, double T, 
 }
 
 // Per-cell coefficients.  F64 storage: always the FP64 evaluation.  FP32 arithmetic: each
-// cell whose FP32 temperature is below ColdPath::T_floor takes an FP32 mobility (zeta = 1,
-// no liquid reset: exact there); every other cell takes the FP64 evaluation.  The choice is
+// cold-path cell (make_cold) takes an FP32 mobility (zeta = 1, no liquid reset: exact
+// there); every other cell takes the FP64 evaluation.  The choice is
 // made per cell from the cell's own temperature, so results do not depend on which cells
 // share a wavefront; the FP64 branch is entered only by wavefronts holding a hot cell.
 template <class TC>
 __device__ __forceinline__ CellCoef<TC> cell_coef(const StepDev& a, const ColdPath& k, int ix, int iy, int iz,
-                                                  long long c, double t, TC S) {
+                                                  long long c, double t, double hstep, TC S) {
   if constexpr (sizeof(TC) == 8) {
     (void)k;
-    return cell_coef64<TC>(a, cell_temperature(a, ix, iy, iz, c, t), S);
+    return cell_coef64<TC>(a, cell_temperature(a, ix, iy, iz, c, t), S, hstep);
   } else {
     const float T32 = cell_temperature_f32(a, k, ix, iy, iz, c);
-    const bool hot = !(T32 < k.T_floor);  // NaN -> FP64 path
     CellCoef<float> r;
     const float Lg = k.L0 * expf(-k.Qg_R / T32);
+    const bool hot = !(T32 < k.T_lim && fabsf(Lg) * k.hs < 1e-6f);  // NaN -> FP64 path
     r.c1 = k.hm * Lg;
     r.c2 = k.hk * Lg;
     r.w1 = r.c2 * k.eps1;
@@ -276,7 +283,7 @@ __device__ __forceinline__ CellCoef<TC> cell_coef(const StepDev& a, const ColdPa
     r.A = fmac(k.gamma4, S, -1.0f);
     r.liquid = false;
     if (__ballot(hot)) {
-      if (hot) r = cell_coef64<float>(a, cell_temperature(a, ix, iy, iz, c, t), S);
+      if (hot) r = cell_coef64<float>(a, cell_temperature(a, ix, iy, iz, c, t), S, hstep);
     }
     return r;
   }
@@ -346,6 +353,21 @@ __device__ __forceinline__ TC update_one(const CellCoef<TC>& cf, TC B, TC ihx2, 
   }
 }
 
+template <class TC> __device__ __forceinline__ TC lap_B(const StepDev& a);
+template <> __device__ __forceinline__ float lap_B<float>(const StepDev& a) { return a.Bf; }
+template <> __device__ __forceinline__ double lap_B<double>(const StepDev& a) { return a.B; }
+template <class TC> __device__ __forceinline__ void inv_h2(const StepDev& a, TC& x, TC& y, TC& z);
+template <> __device__ __forceinline__ void inv_h2<float>(const StepDev& a, float& x, float& y, float& z) {
+  x = a.ihx2f;
+  y = a.ihy2f;
+  z = a.ihz2f;
+}
+template <> __device__ __forceinline__ void inv_h2<double>(const StepDev& a, double& x, double& y, double& z) {
+  x = a.ihx2;
+  y = a.ihy2;
+  z = a.ihz2;
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // ------------------------------------------------------------------ tile geometry
@@ -355,110 +377,78 @@ constexpr int p2div(int n, int cap) {
   return v;
 }
 
-// Channel split of the sum of squares S: for even K >= 16 the register-tiled kernels hold
-// the channels of a cell in two halves (two threads), and every kernel sums S as
-// (sum of the first K/2 squares) + (sum of the last K/2 squares), so all kernels of a K
-// round identically.
-constexpr int split_of(int K) { return K < 0 ? 2 : 1; }  // the two-half split is disabled: one thread per cell
-
-// H threads per cell (H = 2: each of two wavefronts holds half of the K channels of the
-// same 64 cells), KH = K / H channels per thread.
-template <class TS, int K, int H> struct Layout {
+template <class TS, int K> struct Layout {
   using TC = typename Traits<TS>::TC;
-  static_assert(K % H == 0, "H must divide K");
-  static constexpr int KH = K / H;
   // global load chunk (elements): a power of two dividing K, <= 16 bytes of TS and of TC
   static constexpr int VEC = p2div(K, (16 / (int)sizeof(TC)) < (16 / (int)sizeof(TS)) ? 16 / (int)sizeof(TC)
                                                                                         : 16 / (int)sizeof(TS));
-  // LDS read vector of a thread's channel slice (TC elements), and the cell stride KP: a
-  // multiple of LV with KP / LV odd, so consecutive lanes' LV-wide reads hit distinct banks
-  static constexpr int LV = p2div(KH, 16 / (int)sizeof(TC));
+  // LDS read vector (TC elements) and the cell stride KP: a multiple of LV with KP / LV
+  // odd, so consecutive lanes' LV-wide reads hit distinct banks
+  static constexpr int LV = p2div(K, 16 / (int)sizeof(TC)) >= 2 ? 16 / (int)sizeof(TC) : 16 / (int)sizeof(TC);
   static constexpr int KP0 = (K + LV - 1) / LV;
   static constexpr int KP = (KP0 % 2 == 1 ? KP0 : KP0 + 1) * LV;
-  static constexpr int NG = KH / LV;
+  static constexpr int NG = KP0;  // LV-groups read per neighbour (the last may hold padding)
   static constexpr int WV = p2div(KP, VEC);  // LDS write vector of a load chunk (divides KP and VEC)
-  // store vector of a thread's slice (TS elements)
-  static constexpr int SVS = p2div(KH, 16 / (int)sizeof(TS));
+  static constexpr int SVS = p2div(K, 16 / (int)sizeof(TS));  // per-cell store vector (TS elements)
 };
 
-template <class TS, int K, int TX, int TY, int H> struct Tile {
-  using L = Layout<TS, K, H>;
+template <class TS, int K, int TX, int TY> struct Tile {
+  using L = Layout<TS, K>;
   using TC = typename L::TC;
-  static constexpr int NC = TX * TY;             // cells per tile
-  static constexpr int NT = NC * H;              // threads per workgroup
-  static_assert(H == 1 || NC % 64 == 0, "H = 2 needs whole wavefronts of cells");
+  static constexpr int NT = TX * TY;             // threads = cells per workgroup
   static constexpr int PX = TX + 2, PY = TY + 2;
   static constexpr int CPC = K / L::VEC;         // chunks per cell
   static constexpr int NCH = PY * PX * CPC;      // chunks per plane
   static constexpr int CPT = (NCH + NT - 1) / NT;
   static constexpr int PLANE = PY * PX * L::KP;  // TC elements per LDS plane
-  static constexpr int STAGE = PF_STORE_MODE != 0 ? NC * K : 8;  // TS elements of output staging
-  static constexpr int SRING = H > 1 ? 2 * NC * H : 1;          // partial sums of squares, 2 planes
-  static constexpr size_t LDS_BYTES = 2 * PLANE * sizeof(TC) + STAGE * sizeof(TS) + SRING * sizeof(TC);
+  static constexpr int STAGE = PF_STORE_MODE != 0 ? NT * K : 8;  // TS elements of output staging
+  static constexpr size_t LDS_BYTES = 2 * PLANE * sizeof(TC) + STAGE * sizeof(TS);
 };
 
 // ------------------------------------------------------------------ 2.5-D march of one tile
-// A workgroup owns a TX x TY column of cells and marches it upward through z-planes
-// [zb, ze).  Planes (tile + one-cell halo ring, edge-replicated at the domain faces) are
-// double-buffered in LDS; the plane two ahead is prefetched into registers while the
-// current plane is computed.  Each thread keeps its KH channels of its cell at z-1, z, z+1
-// in three register arrays whose roles rotate by renaming (the z loop is unrolled by
-// three).  With H = 2 the two halves of a cell's channels live in two wavefronts; the sum
-// of squares S = sum_j eta_j^2 is completed through an LDS ring written one plane ahead,
-// so it needs no extra barrier.  Results are staged in LDS and written back as contiguous
-// row segments (PF_STORE_MODE 1, 2) or stored per cell (0).
-template <int K, class TS, bool ANISO, int TX, int TY, int H>
+// A workgroup of TX*TY threads owns a TX x TY column of cells and marches it upward through
+// z-planes [zb, ze).  Planes (tile + one-cell halo ring, edge-replicated at the domain faces)
+// are double-buffered in LDS; the plane two ahead is prefetched into registers while the
+// current plane is computed.  Each thread keeps the K channels of its own cell at z-1, z, z+1
+// in three register arrays whose roles rotate by renaming (the z loop is unrolled by three).
+// Results are staged in LDS and written back as contiguous row segments (PF_STORE_MODE 1, 2)
+// or stored per cell (0).  The instance and step arguments are read through references (kernel
+// argument or device memory), so uniform values stay in scalar registers.
+template <int K, class TS, bool ANISO, int TX, int TY>
 struct March {
   using TC = typename Traits<TS>::TC;
-  using L = Layout<TS, K, H>;
-  using G = Tile<TS, K, TX, TY, H>;
-  static constexpr int VEC = L::VEC, KP = L::KP, LV = L::LV, NG = L::NG, KH = L::KH, WV = L::WV;
+  using L = Layout<TS, K>;
+  using G = Tile<TS, K, TX, TY>;
+  static constexpr int VEC = L::VEC, KP = L::KP, LV = L::LV, NG = L::NG, WV = L::WV;
   using SV = Vec<TS, VEC>;
   using WVv = Vec<TC, WV>;
   using LVv = Vec<TC, LV>;
   using SVv = Vec<TS, L::SVS>;
 
-  const StepDev a;
+  const StepDev& a;
+  const StepDyn& d;
   TC* __restrict__ buf0;
   TC* __restrict__ buf1;
-  const TS* __restrict__ in;
-  TS* __restrict__ out;
-  TS* __restrict__ stage;  // LDS staging of one output plane (row-major cells, K each)
-  TC* __restrict__ sring;  // LDS [2][H][NC] partial sums of squares
+  TS* __restrict__ stage;
   int src_off[G::CPT];
   int dst_off[G::CPT];
   SV pre[G::CPT];
-  int cl, h, ch0, gx, gy, own, oxm, oxp, oym, oyp, zb, len;
-  long long plane_elems;
+  int gx, gy, own, oxm, oxp, oym, oyp, zb, len;
   bool active;
-  ColdPath cold;
 
-  __device__ __forceinline__ March(const StepDev& a_, int tile_x, int tile_y, int tile_z, TC* smem, TS* stage_,
-                                   TC* sring_)
-      : a(a_), cold(make_cold(a_, a_.t_prev)) {
+  __device__ __forceinline__ March(const StepDev& a_, const StepDyn& d_, int tile_x, int tile_y, int tile_z,
+                                   TC* smem, TS* stage_)
+      : a(a_), d(d_) {
     const int tid = threadIdx.x;
-    if constexpr (H == 1) {
-      cl = tid;
-      h = 0;
-    } else {
-      const int w = tid >> 6;
-      h = w % H;
-      cl = (w / H) * 64 + (tid & 63);
-    }
-    ch0 = h * KH;
-    const int lx = cl % TX, ly = cl / TX;
+    const int lx = tid % TX, ly = tid / TX;
     gx = tile_x * TX + lx;
     gy = tile_y * TY + ly;
     active = (gx < a.nx) && (gy < a.ny);
     zb = tile_z * a.zc;
     len = min(zb + a.zc, a.nz) - zb;
-    plane_elems = (long long)a.ny * a.nx * K;
-    in = (const TS*)a.in;
-    out = (TS*)a.out;
     buf0 = smem;
     buf1 = smem + G::PLANE;
     stage = stage_;
-    sring = sring_;
 #pragma unroll
     for (int j = 0; j < G::CPT; ++j) {
       const int w = tid + j * G::NT;
@@ -470,15 +460,17 @@ struct March {
       src_off[j] = (sy * a.nx + sx) * K + v * VEC;
       dst_off[j] = (w < G::NCH) ? cs * KP + v * VEC : -1;
     }
-    own = ((ly + 1) * G::PX + (lx + 1)) * KP + ch0;
+    own = ((ly + 1) * G::PX + (lx + 1)) * KP;
     oxm = own - KP;
     oxp = own + KP;
     oym = own - G::PX * KP;
     oyp = own + G::PX * KP;
   }
 
+  __device__ __forceinline__ long long plane_elems() const { return (long long)a.ny * a.nx * K; }
+
   __device__ __forceinline__ void load_plane(int z) {
-    const TS* p = in + (long long)z * plane_elems;
+    const TS* p = (const TS*)d.in + (long long)z * plane_elems();
 #pragma unroll
     for (int j = 0; j < G::CPT; ++j) pre[j] = *(const SV*)(p + src_off[j]);
   }
@@ -496,49 +488,33 @@ struct March {
       }
     }
   }
-  __device__ __forceinline__ void read_center(const TC* b, TC (&dst)[KH]) {
+  __device__ __forceinline__ void read_center(const TC* b, TC (&dst)[K]) {
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
       const LVv v = *(const LVv*)(b + own + g * LV);
 #pragma unroll
-      for (int l = 0; l < LV; ++l) dst[g * LV + l] = v.v[l];
-    }
-  }
-  __device__ __forceinline__ TC part_sq(const TC (&c)[KH]) {
-    TC S = 0;
-#pragma unroll
-    for (int i = 0; i < KH; ++i) S = fmac(c[i], c[i], S);
-    return S;
-  }
-  // the partial sum of squares of this thread's slice of plane zb+n, for the other half
-  __device__ __forceinline__ void put_sq(int n, TC s) {
-    if constexpr (H > 1) sring[((n & 1) * H + h) * G::NC + cl] = s;
-  }
-  // S of plane zb+n: own partial + the other halves' partials, in channel order
-  __device__ __forceinline__ TC full_sq(int n, TC own_part) {
-    if constexpr (H == 1) {
-      return own_part;
-    } else {
-      TC S = 0;
-#pragma unroll
-      for (int q = 0; q < H; ++q) S += (q == h) ? own_part : sring[((n & 1) * H + q) * G::NC + cl];
-      return S;
+      for (int l = 0; l < LV; ++l)
+        if (g * LV + l < K) dst[g * LV + l] = v.v[l];
     }
   }
 
   // Compute plane zb+n from (lo, c, hi) = planes z-1, z, z+1, then (if another plane
   // follows) rotate the LDS ring and read plane z+2 into `lo`.
-  __device__ __forceinline__ void body(int n, TC (&lo)[KH], TC (&c)[KH], TC (&hi)[KH]) {
+  __device__ __forceinline__ void body(int n, TC (&lo)[K], TC (&c)[K], TC (&hi)[K]) {
     const int z = zb + n;
     TC* cb = (n & 1) ? buf1 : buf0;
     if (active) {
       const long long cell = ((long long)z * a.ny + gy) * a.nx + gx;
-      const TC S = full_sq(n, part_sq(c));
-      const CellCoef<TC> cf = cell_coef<TC>(a, cold, gx, gy, z, cell, a.t_prev, S);
-      const TC B = (TC)a.B, ihx2 = (TC)a.ihx2, ihy2 = (TC)a.ihy2, ihz2 = (TC)a.ihz2;
+      TC S = 0;
+#pragma unroll
+      for (int i = 0; i < K; ++i) S = fmac(c[i], c[i], S);
+      const CellCoef<TC> cf = cell_coef<TC>(a, d.cold, gx, gy, z, cell, d.t_prev, d.hstep, S);
+      const TC B = lap_B<TC>(a);
+      TC ihx2, ihy2, ihz2;
+      inv_h2<TC>(a, ihx2, ihy2, ihz2);
       const TC* __restrict__ axes = (const TC*)a.axes;
       if constexpr (ANISO) asm volatile("" : "+s"(axes));  // keep the axis table out of VGPRs: reload per plane
-      TC res[KH];
+      TC res[K];
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         const LVv vxm = *(const LVv*)(cb + oxm + g * LV);
@@ -548,26 +524,28 @@ struct March {
 #pragma unroll
         for (int l = 0; l < LV; ++l) {
           const int i = g * LV + l;
-          // A channel that is exactly zero on the whole stencil of every lane stays
-          // exactly +0 (the full update evaluates to +0 there); skip it wave-wide.
-          if (__ballot(any_nz(c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l], lo[i], hi[i])))
-            res[i] = update_one<TC, ANISO>(cf, B, ihx2, ihy2, ihz2, c[i], vxm.v[l], vxp.v[l], vym.v[l],
-                                           vyp.v[l], lo[i], hi[i], axes, K, ch0 + i);
-          else
-            res[i] = (TC)0;
+          if (i < K) {
+            // A channel that is exactly zero on the whole stencil of every lane stays
+            // exactly +0 (the full update evaluates to +0 there); skip it wave-wide.
+            if (__ballot(any_nz(c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l], lo[i], hi[i])))
+              res[i] = update_one<TC, ANISO>(cf, B, ihx2, ihy2, ihz2, c[i], vxm.v[l], vxp.v[l], vym.v[l],
+                                             vyp.v[l], lo[i], hi[i], axes, K, i);
+            else
+              res[i] = (TC)0;
+          }
         }
       }
       if (__ballot(cf.liquid)) {
 #pragma unroll
-        for (int i = 0; i < KH; ++i) res[i] = cf.liquid ? (TC)0 : res[i];
+        for (int i = 0; i < K; ++i) res[i] = cf.liquid ? (TC)0 : res[i];
       }
 #if PF_STORE_MODE == 0
-      TS* o = out + cell * K + ch0;
+      TS* o = (TS*)d.out + cell * K;
 #else
-      TS* o = stage + cl * K + ch0;
+      TS* o = stage + threadIdx.x * K;
 #endif
 #pragma unroll
-      for (int j = 0; j < KH / L::SVS; ++j) {
+      for (int j = 0; j < K / L::SVS; ++j) {
         SVv s;
 #pragma unroll
         for (int e = 0; e < L::SVS; ++e) s.v[e] = from_tc<TS>(res[j * L::SVS + e]);
@@ -583,7 +561,6 @@ struct March {
       if (n + 2 < len) load_plane(min(z + 3, a.nz - 1));
       __syncthreads();
       read_center(cb, lo);
-      put_sq(n, part_sq(lo));  // plane z+2 uses ring slot n & 1 (its last reader was plane z)
     }
   }
 
@@ -596,9 +573,10 @@ struct March {
     constexpr int ROWB = TX * CELLB;
     constexpr int CPR = ROWB / CH;
     constexpr int NCHO = TY * CPR;
-    const int x0 = gx - cl % TX;
-    const int y0 = gy - cl / TX;
+    const int x0 = gx - (int)(threadIdx.x % TX);
+    const int y0 = gy - (int)(threadIdx.x / TX);
     const int rowb = min(TX, a.nx - x0) * CELLB;
+    TS* out = (TS*)d.out;
 #pragma unroll
     for (int j = 0; j < (NCHO + G::NT - 1) / G::NT; ++j) {
       const int w = threadIdx.x + j * G::NT;
@@ -619,7 +597,7 @@ struct March {
   }
 
   __device__ __forceinline__ void run() {
-    TC r0[KH], r1[KH], r2[KH];
+    TC r0[K], r1[K], r2[K];
     // prologue: planes zb, zb+1 into LDS; plane zb-1 (edge-replicated) straight into registers
     load_plane(zb);
     store_plane(buf0);
@@ -628,9 +606,9 @@ struct March {
     {
       const int zbh = max(zb - 1, 0);
       const int sxo = min(gx, a.nx - 1), syo = min(gy, a.ny - 1);
-      const TS* p = in + (long long)zbh * plane_elems + (long long)(syo * a.nx + sxo) * K + ch0;
+      const TS* p = (const TS*)d.in + (long long)zbh * plane_elems() + (long long)(syo * a.nx + sxo) * K;
 #pragma unroll
-      for (int j = 0; j < KH / L::SVS; ++j) {
+      for (int j = 0; j < K / L::SVS; ++j) {
         const SVv s = *(const SVv*)(p + j * L::SVS);
 #pragma unroll
         for (int e = 0; e < L::SVS; ++e) r0[j * L::SVS + e] = to_tc(s.v[e]);
@@ -639,10 +617,7 @@ struct March {
     __syncthreads();
     read_center(buf0, r1);
     read_center(buf1, r2);
-    put_sq(0, part_sq(r1));
-    put_sq(1, part_sq(r2));
     if (len > 1) load_plane(min(zb + 2, a.nz - 1));
-    __syncthreads();
     for (int n = 0; n < len; n += 3) {
       body(n, r0, r1, r2);
       if (n + 1 < len) body(n + 1, r1, r2, r0);
@@ -651,11 +626,11 @@ struct March {
   }
 };
 
-template <int K, class TS, bool ANISO, int TX, int TY, int H>
-__device__ __forceinline__ void march_tile(const StepDev& a, int tile_x, int tile_y, int tile_z,
-                                           typename Traits<TS>::TC* __restrict__ smem, TS* __restrict__ stage,
-                                           typename Traits<TS>::TC* __restrict__ sring) {
-  March<K, TS, ANISO, TX, TY, H> m(a, tile_x, tile_y, tile_z, smem, stage, sring);
+template <int K, class TS, bool ANISO, int TX, int TY>
+__device__ __forceinline__ void march_tile(const StepDev& a, const StepDyn& d, int tile_x, int tile_y,
+                                           int tile_z, typename Traits<TS>::TC* __restrict__ smem,
+                                           TS* __restrict__ stage) {
+  March<K, TS, ANISO, TX, TY> m(a, d, tile_x, tile_y, tile_z, smem, stage);
   m.run();
 }
 
@@ -664,32 +639,28 @@ __device__ __forceinline__ void march_tile(const StepDev& a, int tile_x, int til
 // K without a register-tiled instantiation.  Same per-cell and per-channel arithmetic
 // as march_tile.  RHS = true writes state_rhs instead of the Euler update.
 template <class TS, bool ANISO, bool RHS = false>
-__device__ __forceinline__ void generic_cell(const StepDev& a, long long cell) {
+__device__ __forceinline__ void generic_cell(const StepDev& a, const StepDyn& dy, long long cell) {
   using TC = typename Traits<TS>::TC;
   const int nx = a.nx, ny = a.ny, nz = a.nz, K = a.k;
   const int ix = (int)(cell % nx);
   const int iy = (int)((cell / nx) % ny);
   const int iz = (int)(cell / ((long long)nx * ny));
-  const TS* __restrict__ in = (const TS*)a.in;
-  TS* __restrict__ out = (TS*)a.out;
+  const TS* __restrict__ in = (const TS*)dy.in;
+  TS* __restrict__ out = (TS*)dy.out;
   const long long sx = K, sy = (long long)nx * K, sz = (long long)nx * ny * K;
   const long long base = cell * K;
   const long long oxm = ix > 0 ? -sx : 0, oxp = ix < nx - 1 ? sx : 0;
   const long long oym = iy > 0 ? -sy : 0, oyp = iy < ny - 1 ? sy : 0;
   const long long ozm = iz > 0 ? -sz : 0, ozp = iz < nz - 1 ? sz : 0;
-  const int H = split_of(K), KH = K / H;
   TC S = 0;
-  for (int q = 0; q < H; ++q) {
-    TC Sq = 0;
-    for (int i = q * KH; i < (q + 1) * KH; ++i) {
-      const TC c = to_tc(in[base + i]);
-      Sq = fmac(c, c, Sq);
-    }
-    S += Sq;
+  for (int i = 0; i < K; ++i) {
+    const TC c = to_tc(in[base + i]);
+    S = fmac(c, c, S);
   }
-  const ColdPath k = make_cold(a, a.t_prev);
-  const CellCoef<TC> cf = cell_coef<TC>(a, k, ix, iy, iz, cell, a.t_prev, S);
-  const TC B = (TC)a.B, ihx2 = (TC)a.ihx2, ihy2 = (TC)a.ihy2, ihz2 = (TC)a.ihz2;
+  const CellCoef<TC> cf = cell_coef<TC>(a, dy.cold, ix, iy, iz, cell, dy.t_prev, dy.hstep, S);
+  const TC B = lap_B<TC>(a);
+  TC ihx2, ihy2, ihz2;
+  inv_h2<TC>(a, ihx2, ihy2, ihz2);
   const TC* __restrict__ axes = (const TC*)a.axes;
   for (int i = 0; i < K; ++i) {
     const long long e = base + i;

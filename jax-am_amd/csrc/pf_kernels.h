@@ -24,64 +24,58 @@ struct WorkItem {
   int tile;
 };
 
-template <int K, class TS, bool ANISO, int TX, int TY, int H = split_of(K)>
-__global__ __launch_bounds__(TX* TY* H) __attribute__((amdgpu_waves_per_eu(2))) void step_kernel(const StepDev a) {
+// Step arguments of slot s at its current step (allen_cahn.py:112  h = t_crt - t_prev).
+__host__ __device__ inline StepDyn slot_dyn(const StepDev& a, const BatchSlot& s) {
+  const double tp = s.t0 + (double)s.n * s.dt;
+  const double tc = s.t0 + (double)(s.n + 1) * s.dt;
+  return make_dyn(a, s.eta[s.parity], s.eta[s.parity ^ 1], tp, tc - tp);
+}
+
+template <int K, class TS, bool ANISO, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) void step_kernel(const StepDev a,
+                                                                                           const StepDyn d) {
   using TC = typename Traits<TS>::TC;
-  using G = Tile<TS, K, TX, TY, H>;
+  using G = Tile<TS, K, TX, TY>;
   __shared__ __attribute__((aligned(16))) TC smem[2 * G::PLANE];
   __shared__ __attribute__((aligned(16))) TS stage[G::STAGE];
-  __shared__ TC sring[G::SRING];
   const int n = a.tiles_x * a.tiles_y * a.tiles_z;
   const int t = xcd_swizzle(blockIdx.x, n);
   const int tx = t % a.tiles_x;
   const int ty = (t / a.tiles_x) % a.tiles_y;
   const int tz = t / (a.tiles_x * a.tiles_y);
-  march_tile<K, TS, ANISO, TX, TY, H>(a, tx, ty, tz, smem, stage, sring);
+  march_tile<K, TS, ANISO, TX, TY>(a, d, tx, ty, tz, smem, stage);
 }
 
 template <class TS, bool ANISO, bool RHS = false>
-__global__ __launch_bounds__(256) void generic_kernel(const StepDev a) {
+__global__ __launch_bounds__(256) void generic_kernel(const StepDev a, const StepDyn d) {
   const long long n = (long long)a.nz * a.ny * a.nx;
   const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell < n) generic_cell<TS, ANISO, RHS>(a, cell);
-}
-
-// Instance view for one slot at its current step.
-__device__ __forceinline__ StepDev slot_view(const StepDev& d, const BatchSlot& s) {
-  StepDev a = d;
-  a.in = s.eta[s.parity];
-  a.out = s.eta[s.parity ^ 1];
-  const double tp = s.t0 + (double)s.n * s.dt;
-  const double tc = s.t0 + (double)(s.n + 1) * s.dt;
-  a.t_prev = tp;
-  a.hstep = tc - tp;
-  return a;
+  if (cell < n) generic_cell<TS, ANISO, RHS>(a, d, cell);
 }
 
 // Persistent batched step: a fixed grid walks the device-resident work list (slot,
 // tile) of one (dtype, K, anisotropy, tile shape) class.  Every workgroup exits after
-// the list is exhausted; slots with no steps left are skipped.
-template <int K, class TS, bool ANISO, int TX, int TY, int H = split_of(K)>
-__global__ __launch_bounds__(TX* TY* H) __attribute__((amdgpu_waves_per_eu(2))) void batch_kernel(const StepDev* __restrict__ descs,
-                                                          const BatchSlot* __restrict__ slots,
-                                                          const WorkItem* __restrict__ work_all,
-                                                          const int2* __restrict__ cinfo, int cls) {
+// the list is exhausted; slots with no steps left are skipped.  descs[slot] and
+// dyns[slot] are read in place (scalar loads), never copied into registers wholesale.
+template <int K, class TS, bool ANISO, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) void batch_kernel(
+    const StepDev* __restrict__ descs, const StepDyn* __restrict__ dyns, const BatchSlot* __restrict__ slots,
+    const WorkItem* __restrict__ work_all, const int2* __restrict__ cinfo, int cls) {
   using TC = typename Traits<TS>::TC;
-  using G = Tile<TS, K, TX, TY, H>;
+  using G = Tile<TS, K, TX, TY>;
   __shared__ __attribute__((aligned(16))) TC smem[2 * G::PLANE];
   __shared__ __attribute__((aligned(16))) TS stage[G::STAGE];
-  __shared__ TC sring[G::SRING];
   const int2 ci = cinfo[cls];  // (offset, count) of this class's work list
   const WorkItem* __restrict__ work = work_all + ci.x;
   for (int w = blockIdx.x; w < ci.y; w += gridDim.x) {
     const WorkItem it = work[w];
-    const BatchSlot s = slots[it.slot];
+    const BatchSlot& s = slots[it.slot];
     if (s.steps_left <= 0 || !s.active) continue;
-    const StepDev a = slot_view(descs[it.slot], s);
+    const StepDev& a = descs[it.slot];
     const int tx = it.tile % a.tiles_x;
     const int ty = (it.tile / a.tiles_x) % a.tiles_y;
     const int tz = it.tile / (a.tiles_x * a.tiles_y);
-    march_tile<K, TS, ANISO, TX, TY, H>(a, tx, ty, tz, smem, stage, sring);
+    march_tile<K, TS, ANISO, TX, TY>(a, dyns[it.slot], tx, ty, tz, smem, stage);
     __syncthreads();  // LDS is reused by the next tile
   }
 }
@@ -89,6 +83,7 @@ __global__ __launch_bounds__(TX* TY* H) __attribute__((amdgpu_waves_per_eu(2))) 
 // Batched generic-K step: work items are (slot, block of 256 cells).
 template <class TS, bool ANISO>
 __global__ __launch_bounds__(256) void batch_generic_kernel(const StepDev* __restrict__ descs,
+                                                           const StepDyn* __restrict__ dyns,
                                                            const BatchSlot* __restrict__ slots,
                                                            const WorkItem* __restrict__ work_all,
                                                            const int2* __restrict__ cinfo, int cls) {
@@ -96,18 +91,18 @@ __global__ __launch_bounds__(256) void batch_generic_kernel(const StepDev* __res
   const WorkItem* __restrict__ work = work_all + ci.x;
   for (int w = blockIdx.x; w < ci.y; w += gridDim.x) {
     const WorkItem it = work[w];
-    const BatchSlot s = slots[it.slot];
+    const BatchSlot& s = slots[it.slot];
     if (s.steps_left <= 0 || !s.active) continue;
-    const StepDev a = slot_view(descs[it.slot], s);
+    const StepDev& a = descs[it.slot];
     const long long n = (long long)a.nz * a.ny * a.nx;
     const long long cell = (long long)it.tile * 256 + threadIdx.x;
-    if (cell < n) generic_cell<TS, ANISO>(a, cell);
+    if (cell < n) generic_cell<TS, ANISO>(a, dyns[it.slot], cell);
   }
 }
 
-using StepLaunch = hipError_t (*)(const StepDev&, hipStream_t);
-using BatchLaunch = hipError_t (*)(const StepDev*, const BatchSlot*, const WorkItem*, const int2*, int cls,
-                                   int grid, hipStream_t);
+using StepLaunch = hipError_t (*)(const StepDev&, const StepDyn&, hipStream_t);
+using BatchLaunch = hipError_t (*)(const StepDev*, const StepDyn*, const BatchSlot*, const WorkItem*, const int2*,
+                                   int cls, int grid, hipStream_t);
 
 struct StepEntry {
   int dtype, k, aniso, tile, tx, ty;
@@ -117,36 +112,36 @@ struct StepEntry {
 };
 
 template <int K, class TS, bool ANISO, int TX, int TY>
-hipError_t launch_step(const StepDev& a, hipStream_t s) {
+hipError_t launch_step(const StepDev& a, const StepDyn& d, hipStream_t s) {
   const int n = a.tiles_x * a.tiles_y * a.tiles_z;
-  step_kernel<K, TS, ANISO, TX, TY><<<dim3(n), dim3(TX * TY * split_of(K)), 0, s>>>(a);
+  step_kernel<K, TS, ANISO, TX, TY><<<dim3(n), dim3(TX * TY), 0, s>>>(a, d);
   return hipGetLastError();
 }
 
 template <int K, class TS, bool ANISO, int TX, int TY>
-hipError_t launch_batch(const StepDev* d, const BatchSlot* sl, const WorkItem* w, const int2* ci, int cls, int grid,
-                        hipStream_t s) {
-  batch_kernel<K, TS, ANISO, TX, TY><<<dim3(grid), dim3(TX * TY * split_of(K)), 0, s>>>(d, sl, w, ci, cls);
+hipError_t launch_batch(const StepDev* a, const StepDyn* d, const BatchSlot* sl, const WorkItem* w, const int2* ci,
+                        int cls, int grid, hipStream_t s) {
+  batch_kernel<K, TS, ANISO, TX, TY><<<dim3(grid), dim3(TX * TY), 0, s>>>(a, d, sl, w, ci, cls);
   return hipGetLastError();
 }
 
 template <class TS, bool ANISO, bool RHS = false>
-hipError_t launch_generic(const StepDev& a, hipStream_t s) {
+hipError_t launch_generic(const StepDev& a, const StepDyn& d, hipStream_t s) {
   const long long n = (long long)a.nz * a.ny * a.nx;
-  generic_kernel<TS, ANISO, RHS><<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(a);
+  generic_kernel<TS, ANISO, RHS><<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(a, d);
   return hipGetLastError();
 }
 
 template <class TS, bool ANISO>
-hipError_t launch_batch_generic(const StepDev* d, const BatchSlot* sl, const WorkItem* w, const int2* ci, int cls,
-                                int grid, hipStream_t s) {
-  batch_generic_kernel<TS, ANISO><<<dim3(grid), dim3(256), 0, s>>>(d, sl, w, ci, cls);
+hipError_t launch_batch_generic(const StepDev* a, const StepDyn* d, const BatchSlot* sl, const WorkItem* w,
+                                const int2* ci, int cls, int grid, hipStream_t s) {
+  batch_generic_kernel<TS, ANISO><<<dim3(grid), dim3(256), 0, s>>>(a, d, sl, w, ci, cls);
   return hipGetLastError();
 }
 
 #define PF_ENTRY(DT, TS, K, A, T, TX, TY)                                                             \
   {DT, K, A, T, TX, TY, &launch_step<K, TS, (A != 0), TX, TY>, &launch_batch<K, TS, (A != 0), TX, TY>, \
-   Tile<TS, K, TX, TY, split_of(K)>::LDS_BYTES}
+   Tile<TS, K, TX, TY>::LDS_BYTES}
 
 // instantiation tables, one per compiled part of pf_inst.hip (see Makefile):
 // parts 0-3 F32, 4-7 BF16, 8-11 F64, each over a K range
