@@ -305,10 +305,11 @@ __device__ __forceinline__ bool any_nz(double c, double a, double b, double d, d
 // the reference's state_rhs value.
 // The anisotropy factor is skipped when |grad eta|^2 == 0 on every lane of the wavefront:
 // there the full evaluation gives q = 1 and ca = w1 exactly, which is what the skip uses.
+// ax: the rotated cube axes of this orientation, axis k at ax[4k .. 4k+2].
 template <class TC, bool ANISO, bool RHS = false>
 __device__ __forceinline__ TC update_one(const CellCoef<TC>& cf, TC B, TC ihx2, TC ihy2, TC ihz2,
                                          TC c, TC xm, TC xp, TC ym, TC yp, TC zm, TC zp,
-                                         const TC* __restrict__ axes, int K, int i) {
+                                         const TC* __restrict__ ax) {
   const TC dx = fmac(c, (TC)-2, xm + xp);
   const TC dy = fmac(c, (TC)-2, ym + yp);
   const TC dz = fmac(c, (TC)-2, zm + zp);
@@ -319,10 +320,9 @@ __device__ __forceinline__ TC update_one(const CellCoef<TC>& cf, TC B, TC ihx2, 
     const TC n2 = fmac(ez, ez, fmac(ey, ey, ex * ex));
     ca = cf.w1;
     if (__ballot(n2 != (TC)0)) {
-      // axes [3][K][3]: axis k of orientation i at (k*K + i)*3
-      const TC* a0 = axes + i * 3;
-      const TC* a1 = axes + (K + i) * 3;
-      const TC* a2 = axes + (2 * K + i) * 3;
+      const TC* a0 = ax;
+      const TC* a1 = ax + 4;
+      const TC* a2 = ax + 8;
       const TC p0 = fmac(a0[2], ez, fmac(a0[1], ey, a0[0] * ex));
       const TC p1 = fmac(a1[2], ez, fmac(a1[1], ey, a1[0] * ex));
       const TC p2 = fmac(a2[2], ez, fmac(a2[1], ey, a2[0] * ex));
@@ -337,9 +337,7 @@ __device__ __forceinline__ TC update_one(const CellCoef<TC>& cf, TC B, TC ihx2, 
       ca = fmac(cf.w2 * q, q - (TC)1, cf.w1);
     }
   } else {
-    (void)axes;
-    (void)K;
-    (void)i;
+    (void)ax;
     ca = cf.c2;
   }
   const TC e2 = c * c;
@@ -369,6 +367,34 @@ template <> __device__ __forceinline__ void inv_h2<double>(const StepDev& a, dou
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also releases global
+// memory, which makes every wave wait for all of its outstanding global loads and stores
+// (s_waitcnt vmcnt(0)) at the barrier; the march's prefetched plane and its row stores
+// must stay in flight across the barriers, and no wave reads global data written by
+// another wave of the same launch (eta_in and eta_out never alias).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Copy the rotated cube axes ([3][K][3] in global memory: axis k of orientation i at
+// (k*K + i)*3) into the LDS layout used by the march: orientation i at ax[12 i], axis k at
+// ax[12 i + 4 k].  zsign = -1 stores the z components negated: a march running downward
+// forms its z difference the other way round (-e_z), and fma(-a_z, -e_z, t) == fma(a_z, e_z, t)
+// exactly, so both directions give bitwise the same projections.  Reads through ordinary
+// per-lane loads once per workgroup; the caller's next barrier publishes the table.
+template <class TC, int K>
+__device__ __forceinline__ void stage_axes(const TC* __restrict__ axes, TC* __restrict__ ax, int tid, int nt,
+                                           int zsign) {
+  for (int w = tid; w < 9 * K; w += nt) {
+    const int i = w / 9, r = w - 9 * (w / 9);
+    const int k = r / 3, j = r - 3 * (r / 3);
+    const TC v = axes[(k * K + i) * 3 + j];
+    ax[12 * i + 4 * k + j] = (j == 2 && zsign < 0) ? -v : v;
+  }
+}
 
 // ------------------------------------------------------------------ tile geometry
 constexpr int p2div(int n, int cap) {
@@ -402,15 +428,21 @@ template <class TS, int K, int TX, int TY> struct Tile {
   static constexpr int CPT = (NCH + NT - 1) / NT;
   static constexpr int PLANE = PY * PX * L::KP;  // TC elements per LDS plane
   static constexpr int STAGE = PF_STORE_MODE != 0 ? NT * K : 8;  // TS elements of output staging
-  static constexpr size_t LDS_BYTES = 2 * PLANE * sizeof(TC) + STAGE * sizeof(TS);
+  static constexpr int AXN = 12 * K;                              // TC elements of the LDS axis table
+  static constexpr size_t LDS_BYTES = 2 * PLANE * sizeof(TC) + STAGE * sizeof(TS) + AXN * sizeof(TC);
 };
 
 // ------------------------------------------------------------------ 2.5-D march of one tile
-// A workgroup of TX*TY threads owns a TX x TY column of cells and marches it upward through
-// z-planes [zb, ze).  Planes (tile + one-cell halo ring, edge-replicated at the domain faces)
-// are double-buffered in LDS; the plane two ahead is prefetched into registers while the
-// current plane is computed.  Each thread keeps the K channels of its own cell at z-1, z, z+1
-// in three register arrays whose roles rotate by renaming (the z loop is unrolled by three).
+// A workgroup of TX*TY threads owns a TX x TY column of cells and marches it through the
+// z-planes [zb, zb + len), upward for even z-chunks and downward for odd ones: two z-adjacent
+// chunks then read their shared boundary planes at the same time (both at their start or both
+// at their end), so the second read is served from L2.  Planes (tile + one-cell halo ring,
+// edge-replicated at the domain faces) are double-buffered in LDS; the plane two ahead is
+// prefetched into registers while the current plane is computed.  Each thread keeps the K
+// channels of its own cell at the planes behind, at and ahead of the current one in three
+// register arrays whose roles rotate by renaming (the z loop is unrolled by three).  The
+// stencil uses behind + ahead (commutative) and ahead - behind as the z difference, whose
+// sign the downward march compensates in the axis table (stage_axes).
 // Results are staged in LDS and written back as contiguous row segments (PF_STORE_MODE 1, 2)
 // or stored per cell (0).  The instance and step arguments are read through references (kernel
 // argument or device memory), so uniform values stay in scalar registers.
@@ -430,14 +462,15 @@ struct March {
   TC* __restrict__ buf0;
   TC* __restrict__ buf1;
   TS* __restrict__ stage;
+  const TC* __restrict__ ax;  // LDS axis table (ANISO)
   int src_off[G::CPT];
   int dst_off[G::CPT];
   SV pre[G::CPT];
-  int gx, gy, own, oxm, oxp, oym, oyp, zb, len;
+  int gx, gy, own, oxm, oxp, oym, oyp, zb, len, dir;
   bool active;
 
   __device__ __forceinline__ March(const StepDev& a_, const StepDyn& d_, int tile_x, int tile_y, int tile_z,
-                                   TC* smem, TS* stage_)
+                                   TC* smem, TS* stage_, TC* ax_)
       : a(a_), d(d_) {
     const int tid = threadIdx.x;
     const int lx = tid % TX, ly = tid / TX;
@@ -446,9 +479,12 @@ struct March {
     active = (gx < a.nx) && (gy < a.ny);
     zb = tile_z * a.zc;
     len = min(zb + a.zc, a.nz) - zb;
+    dir = (tile_z & 1) ? -1 : 1;
     buf0 = smem;
     buf1 = smem + G::PLANE;
     stage = stage_;
+    ax = ax_;
+    if constexpr (ANISO) stage_axes<TC, K>((const TC*)a.axes, ax_, tid, G::NT, dir);
 #pragma unroll
     for (int j = 0; j < G::CPT; ++j) {
       const int w = tid + j * G::NT;
@@ -498,10 +534,14 @@ struct March {
     }
   }
 
-  // Compute plane zb+n from (lo, c, hi) = planes z-1, z, z+1, then (if another plane
+  // plane of march step n, and a plane index moved by d steps along the march (clamped)
+  __device__ __forceinline__ int zplane(int n) const { return dir > 0 ? zb + n : zb + len - 1 - n; }
+  __device__ __forceinline__ int zstep(int z, int d) const { return clampi(z + d * dir, 0, a.nz - 1); }
+
+  // Compute plane z = zplane(n) from (lo, c, hi) = planes behind, at and ahead of z, then (if another plane
   // follows) rotate the LDS ring and read plane z+2 into `lo`.
   __device__ __forceinline__ void body(int n, TC (&lo)[K], TC (&c)[K], TC (&hi)[K]) {
-    const int z = zb + n;
+    const int z = zplane(n);
     TC* cb = (n & 1) ? buf1 : buf0;
     if (active) {
       const long long cell = ((long long)z * a.ny + gy) * a.nx + gx;
@@ -512,8 +552,6 @@ struct March {
       const TC B = lap_B<TC>(a);
       TC ihx2, ihy2, ihz2;
       inv_h2<TC>(a, ihx2, ihy2, ihz2);
-      const TC* __restrict__ axes = (const TC*)a.axes;
-      if constexpr (ANISO) asm volatile("" : "+s"(axes));  // keep the axis table out of VGPRs: reload per plane
       TC res[K];
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
@@ -529,7 +567,7 @@ struct March {
             // exactly +0 (the full update evaluates to +0 there); skip it wave-wide.
             if (__ballot(any_nz(c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l], lo[i], hi[i])))
               res[i] = update_one<TC, ANISO>(cf, B, ihx2, ihy2, ihz2, c[i], vxm.v[l], vxp.v[l], vym.v[l],
-                                             vyp.v[l], lo[i], hi[i], axes, K, i);
+                                             vyp.v[l], lo[i], hi[i], ax + 12 * i);
             else
               res[i] = (TC)0;
           }
@@ -552,14 +590,14 @@ struct March {
         *(SVv*)(o + j * L::SVS) = s;
       }
     }
-    __syncthreads();
+    lds_barrier();
 #if PF_STORE_MODE != 0
     store_rows(z);
 #endif
     if (n + 1 < len) {
       store_plane(cb);  // plane z+2 replaces plane z
-      if (n + 2 < len) load_plane(min(z + 3, a.nz - 1));
-      __syncthreads();
+      if (n + 2 < len) load_plane(zstep(z, 3));
+      lds_barrier();
       read_center(cb, lo);
     }
   }
@@ -598,13 +636,15 @@ struct March {
 
   __device__ __forceinline__ void run() {
     TC r0[K], r1[K], r2[K];
-    // prologue: planes zb, zb+1 into LDS; plane zb-1 (edge-replicated) straight into registers
-    load_plane(zb);
+    // prologue: the first two planes into LDS; the plane behind the first (edge-replicated)
+    // straight into registers
+    const int z0 = zplane(0);
+    load_plane(z0);
     store_plane(buf0);
-    load_plane(min(zb + 1, a.nz - 1));
+    load_plane(zstep(z0, 1));
     store_plane(buf1);
     {
-      const int zbh = max(zb - 1, 0);
+      const int zbh = zstep(z0, -1);
       const int sxo = min(gx, a.nx - 1), syo = min(gy, a.ny - 1);
       const TS* p = (const TS*)d.in + (long long)zbh * plane_elems() + (long long)(syo * a.nx + sxo) * K;
 #pragma unroll
@@ -614,10 +654,10 @@ struct March {
         for (int e = 0; e < L::SVS; ++e) r0[j * L::SVS + e] = to_tc(s.v[e]);
       }
     }
-    __syncthreads();
+    lds_barrier();
     read_center(buf0, r1);
     read_center(buf1, r2);
-    if (len > 1) load_plane(min(zb + 2, a.nz - 1));
+    if (len > 1) load_plane(zstep(z0, 2));
     for (int n = 0; n < len; n += 3) {
       body(n, r0, r1, r2);
       if (n + 1 < len) body(n + 1, r1, r2, r0);
@@ -629,8 +669,8 @@ struct March {
 template <int K, class TS, bool ANISO, int TX, int TY>
 __device__ __forceinline__ void march_tile(const StepDev& a, const StepDyn& d, int tile_x, int tile_y,
                                            int tile_z, typename Traits<TS>::TC* __restrict__ smem,
-                                           TS* __restrict__ stage) {
-  March<K, TS, ANISO, TX, TY> m(a, d, tile_x, tile_y, tile_z, smem, stage);
+                                           TS* __restrict__ stage, typename Traits<TS>::TC* __restrict__ ax) {
+  March<K, TS, ANISO, TX, TY> m(a, d, tile_x, tile_y, tile_z, smem, stage, ax);
   m.run();
 }
 
@@ -663,12 +703,19 @@ __device__ __forceinline__ void generic_cell(const StepDev& a, const StepDyn& dy
   inv_h2<TC>(a, ihx2, ihy2, ihz2);
   const TC* __restrict__ axes = (const TC*)a.axes;
   for (int i = 0; i < K; ++i) {
+    TC ax[12];
+    if constexpr (ANISO) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) ax[4 * k + j] = axes[(k * K + i) * 3 + j];
+    }
     const long long e = base + i;
     const TC c = to_tc(in[e]), xm = to_tc(in[e + oxm]), xp = to_tc(in[e + oxp]), ym = to_tc(in[e + oym]),
              yp = to_tc(in[e + oyp]), zm = to_tc(in[e + ozm]), zp = to_tc(in[e + ozp]);
     TC v;
     if (RHS || __ballot(any_nz(c, xm, xp, ym, yp, zm, zp)))
-      v = update_one<TC, ANISO, RHS>(cf, B, ihx2, ihy2, ihz2, c, xm, xp, ym, yp, zm, zp, axes, K, i);
+      v = update_one<TC, ANISO, RHS>(cf, B, ihx2, ihy2, ihz2, c, xm, xp, ym, yp, zm, zp, ax);
     else
       v = (TC)0;
     if (!RHS && cf.liquid) v = (TC)0;
@@ -677,8 +724,8 @@ __device__ __forceinline__ void generic_cell(const StepDev& a, const StepDyn& dy
 }
 
 // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so give each
-// XCD a contiguous range of logical tiles (x fastest, then y, then z) to keep the
-// halo reuse of neighbouring tiles inside one L2.
+// XCD a contiguous range of logical tiles (z-chunk fastest, then x, then y) to keep the
+// halo reuse of neighbouring tiles, in z as well as in x and y, inside one L2.
 __device__ __forceinline__ int xcd_swizzle(int b, int n) {
   constexpr int NX = 8;
   const int per = n / NX, rem = n % NX;

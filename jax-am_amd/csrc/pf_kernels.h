@@ -38,12 +38,14 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) voi
   using G = Tile<TS, K, TX, TY>;
   __shared__ __attribute__((aligned(16))) TC smem[2 * G::PLANE];
   __shared__ __attribute__((aligned(16))) TS stage[G::STAGE];
+  __shared__ __attribute__((aligned(16))) TC ax[ANISO ? G::AXN : 4];
   const int n = a.tiles_x * a.tiles_y * a.tiles_z;
   const int t = xcd_swizzle(blockIdx.x, n);
-  const int tx = t % a.tiles_x;
-  const int ty = (t / a.tiles_x) % a.tiles_y;
-  const int tz = t / (a.tiles_x * a.tiles_y);
-  march_tile<K, TS, ANISO, TX, TY>(a, d, tx, ty, tz, smem, stage);
+  // z fastest: the z-chunks of one column run together on one XCD
+  const int tz = t % a.tiles_z;
+  const int tx = (t / a.tiles_z) % a.tiles_x;
+  const int ty = t / (a.tiles_z * a.tiles_x);
+  march_tile<K, TS, ANISO, TX, TY>(a, d, tx, ty, tz, smem, stage, ax);
 }
 
 template <class TS, bool ANISO, bool RHS = false>
@@ -65,6 +67,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) voi
   using G = Tile<TS, K, TX, TY>;
   __shared__ __attribute__((aligned(16))) TC smem[2 * G::PLANE];
   __shared__ __attribute__((aligned(16))) TS stage[G::STAGE];
+  __shared__ __attribute__((aligned(16))) TC ax[ANISO ? G::AXN : 4];
   const int2 ci = cinfo[cls];  // (offset, count) of this class's work list
   const WorkItem* __restrict__ work = work_all + ci.x;
   for (int w = blockIdx.x; w < ci.y; w += gridDim.x) {
@@ -72,11 +75,11 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) voi
     const BatchSlot& s = slots[it.slot];
     if (s.steps_left <= 0 || !s.active) continue;
     const StepDev& a = descs[it.slot];
-    const int tx = it.tile % a.tiles_x;
-    const int ty = (it.tile / a.tiles_x) % a.tiles_y;
-    const int tz = it.tile / (a.tiles_x * a.tiles_y);
-    march_tile<K, TS, ANISO, TX, TY>(a, dyns[it.slot], tx, ty, tz, smem, stage);
-    __syncthreads();  // LDS is reused by the next tile
+    const int tz = it.tile % a.tiles_z;
+    const int tx = (it.tile / a.tiles_z) % a.tiles_x;
+    const int ty = it.tile / (a.tiles_z * a.tiles_x);
+    march_tile<K, TS, ANISO, TX, TY>(a, dyns[it.slot], tx, ty, tz, smem, stage, ax);
+    lds_barrier();  // LDS is reused by the next tile
   }
 }
 
