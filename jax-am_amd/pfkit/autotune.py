@@ -2,24 +2,33 @@
 
 A candidate is (tile, zchunk): the workgroup's (TX, TY) column shape (pf_tile_shape)
 and the number of z-planes one workgroup marches.  ``tune`` times every candidate with
-HIP events on the real problem size and stores the fastest per key
-(dtype, K, anisotropy, shape class) in a JSON cache next to this file (or at
-$PFKIT_AUTOTUNE_CACHE).  ``lookup`` returns the cached choice, else the library
-defaults (-1, 0).  Shape class: the grid (Nz, Ny, Nx) rounded up to powers of two, so
-nearby sizes share an entry.
+HIP events on the real problem size, on a state that has already been advanced a few
+steps (the kernel skips all-zero channels per wavefront, so a one-hot start would time
+an unrepresentative amount of work), and records the fastest per key (dtype, K,
+anisotropy, shape class).  Shape class: the grid (Nz, Ny, Nx) rounded up to powers of
+two, so nearby sizes share an entry.
+
+``lookup`` consults, in order, the user cache ($PFKIT_AUTOTUNE_CACHE, default
+``_autotune_cache.json`` next to this file; written by ``tune``) and the table shipped
+with the library (``tiles_mi355x.json``, produced on an MI355X by
+``python -m pfkit.autotune --sweep``), else returns the library defaults (-1, 0).
 """
+import argparse
 import json
 import os
+import sys
 import threading
 
 import torch
 
 from . import _lib
 
-_CACHE_PATH = os.environ.get("PFKIT_AUTOTUNE_CACHE",
-                             os.path.join(os.path.dirname(os.path.abspath(__file__)), "_autotune_cache.json"))
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_CACHE_PATH = os.environ.get("PFKIT_AUTOTUNE_CACHE", os.path.join(_HERE, "_autotune_cache.json"))
+_TABLE_PATH = os.path.join(_HERE, "tiles_mi355x.json")
 _lock = threading.Lock()
 _cache = None
+_table = None
 
 
 def _p2(n):
@@ -34,21 +43,29 @@ def key(pf_dtype, K, aniso, Nz, Ny, Nx):
     return f"d{pf_dtype}_k{int(K)}_a{int(bool(aniso))}_{_p2(Nz)}x{_p2(Ny)}x{_p2(Nx)}"
 
 
+def _read(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
 def _load():
-    global _cache
+    global _cache, _table
     if _cache is None:
-        try:
-            with open(_CACHE_PATH) as f:
-                _cache = json.load(f)
-        except (OSError, ValueError):
-            _cache = {}
-    return _cache
+        _cache = _read(_CACHE_PATH)
+    if _table is None:
+        _table = _read(_TABLE_PATH).get("entries", {})
+    return _cache, _table
 
 
 def lookup(pf_dtype, pf_args, aniso):
+    """(tile, zchunk) for this problem: user cache, then the shipped table, then (-1, 0)."""
+    k = key(pf_dtype, pf_args["num_oris"], aniso, pf_args["Nz"], pf_args["Ny"], pf_args["Nx"])
     with _lock:
-        c = _load()
-        e = c.get(key(pf_dtype, pf_args["num_oris"], aniso, pf_args["Nz"], pf_args["Ny"], pf_args["Nx"]))
+        cache, table = _load()
+        e = cache.get(k) or table.get(k)
     if e is None:
         return -1, 0
     return int(e["tile"]), int(e["zchunk"])
@@ -57,35 +74,112 @@ def lookup(pf_dtype, pf_args, aniso):
 def candidates(pf_dtype, K, Nz):
     lib = _lib.load()
     nt = lib.pf_tile_count(pf_dtype, K)
-    zcs = sorted({z for z in (0, 4, 8, 11, 16, 22, 32, Nz) if z <= Nz})
+    zcs = sorted({z for z in (0, 4, 8, 12, 16, 24, 32, Nz) if z <= Nz})
     return [(t, z) for t in range(max(nt, 1)) for z in zcs]
 
 
-def tune(solver_rhs, eta, T, t_prev, dt, reps=20, warmup=3, store=True):
-    """Time every candidate for this instance; returns (best (tile, zchunk), {cand: ms})."""
+def time_candidates(solver_rhs, eta, T, t_prev, dt, reps=20, warmup=3):
+    """{(tile, zchunk): ms per step} for every candidate on this state."""
     out = torch.empty_like(eta)
     times = {}
+    stream = torch.cuda.current_stream()
     for cand in candidates(solver_rhs.pf_dtype, solver_rhs.K, int(solver_rhs.pf_args["Nz"])):
         tile, zc = cand
         for _ in range(warmup):
             solver_rhs.step(eta, out, T, t_prev, t_prev + dt, tile, zc)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
+        e0.record(stream)
         for _ in range(reps):
             solver_rhs.step(eta, out, T, t_prev, t_prev + dt, tile, zc)
-        e1.record()
+        e1.record(stream)
         e1.synchronize()
         times[cand] = e0.elapsed_time(e1) / reps
+    return times
+
+
+def tune(solver_rhs, eta, T, t_prev, dt, reps=20, warmup=3, store=True):
+    """Time every candidate for this instance; returns (best (tile, zchunk), {cand: ms}).
+    With store=True the choice goes into the user cache."""
+    times = time_candidates(solver_rhs, eta, T, t_prev, dt, reps, warmup)
     best = min(times, key=times.get)
     if store:
         a = solver_rhs.pf_args
         with _lock:
-            c = _load()
-            c[key(solver_rhs.pf_dtype, solver_rhs.K, solver_rhs.aniso, a["Nz"], a["Ny"], a["Nx"])] = dict(
+            cache, _ = _load()
+            cache[key(solver_rhs.pf_dtype, solver_rhs.K, solver_rhs.aniso, a["Nz"], a["Ny"], a["Nx"])] = dict(
                 tile=best[0], zchunk=best[1], ms=times[best])
             try:
                 with open(_CACHE_PATH, "w") as f:
-                    json.dump(c, f, indent=1, sort_keys=True)
+                    json.dump(cache, f, indent=1, sort_keys=True)
             except OSError:
                 pass
     return best, times
+
+
+# ---------------------------------------------------------------- table sweep
+# Shape classes of the benchmark workloads: the 256x256x64 grid, and the classes that grids
+# drawn log-uniformly between 32^3 and 256x256x64 with Nx = Ny fall into.  Each class is
+# tuned on a grid of 3/4 of its upper bound per axis (at least 32).
+SWEEP_CLASSES = [(64, 256, 256), (32, 32, 32), (64, 64, 64), (64, 128, 128)]
+
+
+def _dtype(name):
+    return {"float32": torch.float32, "bfloat16": torch.bfloat16, "float64": torch.float64}[name]
+
+
+def _tune_one(dtype, K, aniso, Nz, Ny, Nx, steps_before=20, log=None):
+    from . import workloads
+    from .allen_cahn import PFSolver
+    wl = workloads.make("tune", Nx, Ny, Nz, K, max(8, Nx * Ny * Nz // 8000), 1000, dtype, 0.15 if aniso else 0.0)
+    s = PFSolver(wl.pf_args, wl.polycrystal, dtype=dtype, clean=False, tile=-1, zchunk=0)
+    ts = wl.ts(steps_before + 1)
+    eta = s.run(s.ini_cond(), ts[:steps_before + 1], wl.laser)
+    times = time_candidates(s.state_rhs, eta, wl.laser, ts[steps_before], wl.dt)
+    best = min(times, key=times.get)
+    k = key(s.state_rhs.pf_dtype, K, aniso, Nz, Ny, Nx)
+    entry = dict(tile=best[0], zchunk=best[1], ms=round(times[best], 5), default_ms=round(times[(0, 0)], 5),
+                 tuned_on=[Nz, Ny, Nx])
+    if log:
+        log(f"{k}: tile {best[0]} zchunk {best[1]} {times[best]:.4f} ms (default {times[(0, 0)]:.4f} ms)")
+    return k, entry
+
+
+def sweep(out_path, Ks=(10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20), log=None):
+    """Tune the shipped table: c1/c2/FP64 shapes plus the batched/serving classes."""
+    entries = {}
+    jobs = [("float32", 20, True, 64, 256, 256), ("bfloat16", 20, True, 64, 256, 256),
+            ("float64", 20, True, 64, 256, 256), ("float32", 20, False, 64, 256, 256),
+            ("float64", 10, False, 32, 64, 64)]
+    exact = len(jobs)
+    for (Nz, Ny, Nx) in SWEEP_CLASSES:
+        for K in Ks:
+            for dtype in ("float32", "bfloat16"):
+                jobs.append((dtype, K, True, Nz, Ny, Nx))
+    for j, (dtype, K, aniso, Nz, Ny, Nx) in enumerate(jobs):
+        if j >= exact:
+            Nz, Ny, Nx = (max(32, (3 * v) // 4) for v in (Nz, Ny, Nx))
+        if key(_lib.TORCH_TO_PF[_dtype(dtype)], K, aniso, Nz, Ny, Nx) in entries:
+            continue
+        k, e = _tune_one(dtype, K, aniso, Nz, Ny, Nx, log=log)
+        entries[k] = e
+    doc = {"device": torch.cuda.get_device_name(0), "library_abi": int(_lib.load().pf_version()),
+           "method": "fastest of all (tile, zchunk) candidates, 20 timed steps after 3 warm-up steps, on the "
+                     "state reached after 20 steps of the workload's one-hot start",
+           "entries": entries}
+    with open(out_path, "w") as f:
+        json.dump(doc, f, indent=1, sort_keys=True)
+    return doc
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="tile autotuner of the fused step")
+    ap.add_argument("--sweep", action="store_true", help="tune the shipped table's shape classes")
+    ap.add_argument("--out", default=_TABLE_PATH)
+    args = ap.parse_args(argv)
+    if args.sweep:
+        torch.cuda.set_device(0)
+        sweep(args.out, log=lambda *a: print(*a, file=sys.stderr, flush=True))
+
+
+if __name__ == "__main__":
+    main()

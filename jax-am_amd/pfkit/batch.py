@@ -34,8 +34,8 @@ class Instance:
     t0: float = 0.0
     cell_ori_inds: object = None
     eta0: object = None
-    tile: int = -1
-    zchunk: int = 0
+    tile: object = None             # None: the autotuner's choice (pfkit.autotune.lookup)
+    zchunk: object = None
     tag: object = None
     # filled by the engine
     slot: int = -1
@@ -96,6 +96,11 @@ class BatchEngine:
         bi.T_dev = None
         bi.axes_dev = axes.data_ptr() if axes is not None else None
         bi.dtype = _lib.TORCH_TO_PF[tdt]
+        if inst.tile is None or inst.zchunk is None:
+            from . import autotune
+            t, z = autotune.lookup(_lib.TORCH_TO_PF[tdt], a, float(a["anisotropy"]) > 0)
+            inst.tile = t if inst.tile is None else inst.tile
+            inst.zchunk = z if inst.zchunk is None else inst.zchunk
         bi.tile = int(inst.tile)
         bi.zchunk = int(inst.zchunk)
         bi.steps = int(inst.steps)

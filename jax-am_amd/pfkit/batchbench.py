@@ -19,7 +19,7 @@ def run_batched(log, n=64, chunk=20, seed=3):
     updates = sum(w.N * w.K * w.steps for w in insts)
     solvers, eta0s = [], []
     for w in insts:
-        s = PFSolver(w.pf_args, w.polycrystal, dtype="float32", clean=False, tile=-1, zchunk=0)
+        s = PFSolver(w.pf_args, w.polycrystal, dtype="float32", clean=False)
         solvers.append(s)
         eta0s.append(s.ini_cond())
     torch.cuda.synchronize()
@@ -29,13 +29,13 @@ def run_batched(log, n=64, chunk=20, seed=3):
     t_alone = time.perf_counter() - t0
     log(f"  c3 one-at-a-time: {t_alone:.3f} s")
     eng = BatchEngine(max_instances=n + 8)
-    slots = [eng.add(w.instance(tile=-1, zchunk=0)) for w in insts]
+    slots = [eng.add(w.instance()) for w in insts]
     eng.stream.synchronize()
     eng.run(chunk, chunk)               # warm-up: graph capture for this class set, then reset
     eng.stream.synchronize()
     for sl in slots:
         eng.remove(sl)
-    slots = [eng.add(w.instance(tile=-1, zchunk=0)) for w in insts]
+    slots = [eng.add(w.instance()) for w in insts]
     eng.stream.synchronize()
     t0 = time.perf_counter()
     eng.run_to_completion(chunk)
