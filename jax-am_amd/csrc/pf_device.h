@@ -26,6 +26,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "pfhip.h"
 
 #ifndef PF_STORE_MODE
@@ -56,10 +58,16 @@ template <> struct Traits<bf16_t> {
 template <class T, int N> struct alignas(sizeof(T) * N) Vec {
   T v[N];
 };
+// the same with a given alignment (global chunks that start on 4- or 8-byte boundaries)
+template <class T, int N, int A> struct alignas(A) VecA {
+  T v[N];
+};
 
 // raw vector type of a 2/4/8/16-byte piece
 template <int BYTES> struct RawT;
 template <> struct RawT<16> { typedef unsigned int type __attribute__((ext_vector_type(4))); };
+// 16-byte piece at a 2-byte-aligned global address (global accesses run in unaligned mode)
+typedef unsigned int raw16a2_t __attribute__((ext_vector_type(4), aligned(2)));
 template <> struct RawT<8> { typedef unsigned int type __attribute__((ext_vector_type(2))); };
 template <> struct RawT<4> { typedef unsigned int type; };
 template <> struct RawT<2> { typedef unsigned short type; };
@@ -405,9 +413,16 @@ constexpr int p2div(int n, int cap) {
 
 template <class TS, int K> struct Layout {
   using TC = typename Traits<TS>::TC;
-  // global load chunk (elements): a power of two dividing K, <= 16 bytes of TS and of TC
-  static constexpr int VEC = p2div(K, (16 / (int)sizeof(TC)) < (16 / (int)sizeof(TS)) ? 16 / (int)sizeof(TC)
-                                                                                        : 16 / (int)sizeof(TS));
+  // widest global chunk: 16 bytes of TS and of TC
+  static constexpr int VCAP = (16 / (int)sizeof(TC)) < (16 / (int)sizeof(TS)) ? 16 / (int)sizeof(TC)
+                                                                              : 16 / (int)sizeof(TS);
+  // K not a multiple of VCAP: a cell is loaded as K / VCAP chunks of VCAP elements plus one
+  // tail chunk of K % VCAP elements, at the cell's own alignment (GA = sizeof(TS) bytes;
+  // gfx950 global accesses run in unaligned mode, so these stay single dwordx2/x3/x4 loads).
+  static constexpr bool SPLIT = K % VCAP != 0 && K > VCAP;
+  static constexpr int VEC = SPLIT ? VCAP : p2div(K, VCAP);
+  static constexpr int TAIL = SPLIT ? K % VCAP : 0;
+  static constexpr int GA = SPLIT ? (int)sizeof(TS) : VEC * (int)sizeof(TS);
   // LDS read vector (TC elements) and the cell stride KP: a multiple of LV with KP / LV
   // odd, so consecutive lanes' LV-wide reads hit distinct banks
   static constexpr int LV = p2div(K, 16 / (int)sizeof(TC)) >= 2 ? 16 / (int)sizeof(TC) : 16 / (int)sizeof(TC);
@@ -423,9 +438,11 @@ template <class TS, int K, int TX, int TY> struct Tile {
   using TC = typename L::TC;
   static constexpr int NT = TX * TY;             // threads = cells per workgroup
   static constexpr int PX = TX + 2, PY = TY + 2;
-  static constexpr int CPC = K / L::VEC;         // chunks per cell
-  static constexpr int NCH = PY * PX * CPC;      // chunks per plane
+  static constexpr int CPC = K / L::VEC;         // full chunks per cell
+  static constexpr int NCH = PY * PX * CPC;      // full chunks per plane
   static constexpr int CPT = (NCH + NT - 1) / NT;
+  static constexpr int NCT = L::TAIL ? PY * PX : 0;          // tail chunks per plane (one per cell)
+  static constexpr int CPTT = (NCT + NT - 1) / NT;
   static constexpr int PLANE = PY * PX * L::KP;  // TC elements per LDS plane
   static constexpr int STAGE = PF_STORE_MODE != 0 ? NT * K : 8;  // TS elements of output staging
   static constexpr int AXN = 12 * K;                              // TC elements of the LDS axis table
@@ -451,8 +468,10 @@ struct March {
   using TC = typename Traits<TS>::TC;
   using L = Layout<TS, K>;
   using G = Tile<TS, K, TX, TY>;
-  static constexpr int VEC = L::VEC, KP = L::KP, LV = L::LV, NG = L::NG, WV = L::WV;
-  using SV = Vec<TS, VEC>;
+  static constexpr int VEC = L::VEC, KP = L::KP, LV = L::LV, NG = L::NG, WV = L::WV, TAIL = L::TAIL;
+  static constexpr int TV = TAIL > 0 ? TAIL : 1;
+  using SV = VecA<TS, VEC, L::GA>;
+  using TLv = VecA<TS, TV, L::GA>;
   using WVv = Vec<TC, WV>;
   using LVv = Vec<TC, LV>;
   using SVv = Vec<TS, L::SVS>;
@@ -466,6 +485,9 @@ struct March {
   int src_off[G::CPT];
   int dst_off[G::CPT];
   SV pre[G::CPT];
+  int tsrc_off[G::CPTT > 0 ? G::CPTT : 1];
+  int tdst_off[G::CPTT > 0 ? G::CPTT : 1];
+  TLv pret[G::CPTT > 0 ? G::CPTT : 1];
   int gx, gy, own, oxm, oxp, oym, oyp, zb, len, dir;
   bool active;
 
@@ -496,6 +518,18 @@ struct March {
       src_off[j] = (sy * a.nx + sx) * K + v * VEC;
       dst_off[j] = (w < G::NCH) ? cs * KP + v * VEC : -1;
     }
+    if constexpr (TAIL > 0) {
+#pragma unroll
+      for (int j = 0; j < G::CPTT; ++j) {
+        const int w = tid + j * G::NT;
+        const int cs = w < G::NCT ? w : G::NCT - 1;
+        const int r = cs / G::PX, cc = cs % G::PX;
+        const int sy = clampi(tile_y * TY - 1 + r, 0, a.ny - 1);
+        const int sx = clampi(tile_x * TX - 1 + cc, 0, a.nx - 1);
+        tsrc_off[j] = (sy * a.nx + sx) * K + G::CPC * VEC;
+        tdst_off[j] = (w < G::NCT) ? cs * KP + G::CPC * VEC : -1;
+      }
+    }
     own = ((ly + 1) * G::PX + (lx + 1)) * KP;
     oxm = own - KP;
     oxp = own + KP;
@@ -509,6 +543,10 @@ struct March {
     const TS* p = (const TS*)d.in + (long long)z * plane_elems();
 #pragma unroll
     for (int j = 0; j < G::CPT; ++j) pre[j] = *(const SV*)(p + src_off[j]);
+    if constexpr (TAIL > 0) {
+#pragma unroll
+      for (int j = 0; j < G::CPTT; ++j) pret[j] = *(const TLv*)(p + tsrc_off[j]);
+    }
   }
   __device__ __forceinline__ void store_plane(TC* b) {
 #pragma unroll
@@ -520,6 +558,15 @@ struct March {
 #pragma unroll
           for (int e = 0; e < WV; ++e) c.v[e] = to_tc(pre[j].v[q * WV + e]);
           *(WVv*)(b + dst_off[j] + q * WV) = c;
+        }
+      }
+    }
+    if constexpr (TAIL > 0) {
+#pragma unroll
+      for (int j = 0; j < G::CPTT; ++j) {
+        if (tdst_off[j] >= 0) {
+#pragma unroll
+          for (int e = 0; e < TAIL; ++e) b[tdst_off[j] + e] = to_tc(pret[j].v[e]);
         }
       }
     }
@@ -606,9 +653,17 @@ struct March {
   // lanes store consecutive CH-byte pieces of each contiguous (y, z) row segment.
   __device__ __forceinline__ void store_rows(int z) {
     constexpr int CELLB = K * (int)sizeof(TS);
-    constexpr int CH = (CELLB & -CELLB) >= 16 ? 16 : (CELLB & -CELLB);
-    using CT = typename RawT<CH>::type;
     constexpr int ROWB = TX * CELLB;
+    // rows whose full length is a multiple of 16 bytes go out in 16-byte pieces whatever the
+    // cell alignment (a partial row's last piece in units of U bytes); otherwise the widest
+    // power-of-two piece dividing a cell
+    constexpr int CA = CELLB & -CELLB;
+    constexpr bool WIDE = ROWB % 16 == 0 && CA < 16;
+    constexpr int CH = WIDE ? 16 : (CA >= 16 ? 16 : CA);
+    constexpr int U = CA < 4 ? CA : 4;
+    using CT = typename RawT<CH>::type;
+    using GT = typename std::conditional<WIDE, raw16a2_t, CT>::type;
+    using UT = typename RawT<U>::type;
     constexpr int CPR = ROWB / CH;
     constexpr int NCHO = TY * CPR;
     const int x0 = gx - (int)(threadIdx.x % TX);
@@ -622,13 +677,18 @@ struct March {
         const int r = w / CPR;
         const int off = (w - r * CPR) * CH;
         if (y0 + r < a.ny && off < rowb) {
-          const CT v = *(const CT*)((const char*)stage + r * ROWB + off);
+          const char* src = (const char*)stage + r * ROWB + off;
           char* dst = (char*)(out + (((long long)z * a.ny + y0 + r) * a.nx + x0) * K) + off;
+          if (!WIDE || off + CH <= rowb) {
+            const GT v = *(const CT*)src;
 #if PF_STORE_MODE == 2
-          __builtin_nontemporal_store(v, (CT*)dst);
+            __builtin_nontemporal_store(v, (GT*)dst);
 #else
-          *(CT*)dst = v;
+            *(GT*)dst = v;
 #endif
+          } else {
+            for (int q = 0; q < rowb - off; q += U) *(UT*)(dst + q) = *(const UT*)(src + q);
+          }
         }
       }
     }
@@ -648,10 +708,15 @@ struct March {
       const int sxo = min(gx, a.nx - 1), syo = min(gy, a.ny - 1);
       const TS* p = (const TS*)d.in + (long long)zbh * plane_elems() + (long long)(syo * a.nx + sxo) * K;
 #pragma unroll
-      for (int j = 0; j < K / L::SVS; ++j) {
-        const SVv s = *(const SVv*)(p + j * L::SVS);
+      for (int j = 0; j < G::CPC; ++j) {
+        const SV s = *(const SV*)(p + j * VEC);
 #pragma unroll
-        for (int e = 0; e < L::SVS; ++e) r0[j * L::SVS + e] = to_tc(s.v[e]);
+        for (int e = 0; e < VEC; ++e) r0[j * VEC + e] = to_tc(s.v[e]);
+      }
+      if constexpr (TAIL > 0) {
+        const TLv s = *(const TLv*)(p + G::CPC * VEC);
+#pragma unroll
+        for (int e = 0; e < TAIL; ++e) r0[G::CPC * VEC + e] = to_tc(s.v[e]);
       }
     }
     lds_barrier();

@@ -51,6 +51,8 @@ CASES = [
     (40, 9, 13, 3, 0.3, "quench", True),
     (7, 5, 6, 5, 0.15, "rosen", True),
     (24, 20, 12, 16, 0.0, "gauss", True),
+    (21, 11, 9, 17, 0.15, "gauss", True),    # odd K: 16-byte chunks + a 1-element tail, partial rows
+    (35, 13, 7, 19, 0.0, "quench", False),   # odd K, 3-element tail
     (12, 10, 8, 25, 0.15, "gauss", False),   # generic-K kernel
     (9, 7, 5, 33, 0.0, "quench", False),     # generic-K kernel
     (1, 1, 12, 4, 0.15, "gauss", False),     # degenerate 1 x 1 x Nz column
@@ -105,8 +107,9 @@ def test_bf16_step_matches_oracle(case, cuda):
     assert np.all(np.abs(got - ref) <= 2.0 ** -8 * np.abs(ref) + 2e-6)
 
 
-def test_tile_shapes_and_generic_kernel_agree_bitwise(cuda):
-    Nx, Ny, Nz, K = 37, 21, 19, 10
+@pytest.mark.parametrize("K", [10, 17])
+def test_tile_shapes_and_generic_kernel_agree_bitwise(K, cuda):
+    Nx, Ny, Nz = 37, 21, 19
     a = args(Nx, Ny, Nz, K, anisotropy=0.15, force_liquid=True)
     pc = poly(Nx, Ny, Nz, K)
     L = gaussian(Nx, Ny, Nz)
