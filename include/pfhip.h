@@ -124,9 +124,11 @@ int pf_step(const void* eta_in_dev, void* eta_out_dev, int32_t dtype, const void
 int pf_rhs(const void* eta_in_dev, void* rhs_out_dev, int32_t dtype, const void* T_dev,
            const pf_params_t* p, const void* axes_dev, double t, void* stream);
 
-/* Tile shapes available for (dtype, K): count, and (TX, TY) of shape `tile`. */
+/* Tile shapes available for (dtype, K): count, (TX, TY) cells of shape `tile`, and the
+ * threads per cell of shape `tile` (1, or 2 when a lane pair splits the K channels). */
 int pf_tile_count(int32_t dtype, int32_t k);
 int pf_tile_shape(int32_t dtype, int32_t k, int32_t tile, int32_t* tx, int32_t* ty);
+int pf_tile_lanes(int32_t dtype, int32_t k, int32_t tile);
 /* 1 if (dtype, K) runs on the register-tiled fused kernel, 0 if on the generic-K kernel. */
 int pf_fast_path(int32_t dtype, int32_t k);
 

@@ -488,6 +488,12 @@ int pf_tile_shape(int32_t dtype, int32_t k, int32_t tile, int32_t* tx, int32_t* 
   return PF_OK;
 }
 
+int pf_tile_lanes(int32_t dtype, int32_t k, int32_t tile) {
+  const StepEntry* e = find_entry(dtype, k, false, tile);
+  if (!e) return fail(PF_E_ARG, "no such tile shape for this (dtype, K)");
+  return e->h;
+}
+
 int pf_fast_path(int32_t dtype, int32_t k) { return entry_count(dtype, k, false) > 0 ? 1 : 0; }
 
 int pf_step(const void* eta_in_dev, void* eta_out_dev, int32_t dtype, const void* T_dev, const pf_params_t* p,

@@ -389,18 +389,14 @@ __device__ __forceinline__ void lds_barrier() {
 
 // Copy the rotated cube axes ([3][K][3] in global memory: axis k of orientation i at
 // (k*K + i)*3) into the LDS layout used by the march: orientation i at ax[12 i], axis k at
-// ax[12 i + 4 k].  zsign = -1 stores the z components negated: a march running downward
-// forms its z difference the other way round (-e_z), and fma(-a_z, -e_z, t) == fma(a_z, e_z, t)
-// exactly, so both directions give bitwise the same projections.  Reads through ordinary
-// per-lane loads once per workgroup; the caller's next barrier publishes the table.
+// ax[12 i + 4 k].  Reads through ordinary per-lane loads once per workgroup; the caller's
+// next barrier publishes the table.
 template <class TC, int K>
-__device__ __forceinline__ void stage_axes(const TC* __restrict__ axes, TC* __restrict__ ax, int tid, int nt,
-                                           int zsign) {
+__device__ __forceinline__ void stage_axes(const TC* __restrict__ axes, TC* __restrict__ ax, int tid, int nt) {
   for (int w = tid; w < 9 * K; w += nt) {
     const int i = w / 9, r = w - 9 * (w / 9);
     const int k = r / 3, j = r - 3 * (r / 3);
-    const TC v = axes[(k * K + i) * 3 + j];
-    ax[12 * i + 4 * k + j] = (j == 2 && zsign < 0) ? -v : v;
+    ax[12 * i + 4 * k + j] = axes[(k * K + i) * 3 + j];
   }
 }
 
@@ -411,8 +407,37 @@ constexpr int p2div(int n, int cap) {
   return v;
 }
 
-template <class TS, int K> struct Layout {
+// Bank conflicts (extra LDS cycles) of one wavefront's neighbour reads: lane l reads LV
+// elements of EB bytes at cell (l / H) of a TX-wide tile row, channel offset (l % H) * KH,
+// cell stride KP elements.  NB banks of 4 bytes; a read of W bytes per lane is served
+// 4 * NB / W lanes at a time.
+constexpr int lds_conflicts(int KP, int KH, int LV, int H, int TX, int EB, int NB) {
+  const int W = LV * EB;
+  int per = (4 * NB) / W;
+  if (per < 1) per = 1;
+  if (per > 64) per = 64;
+  int total = 0;
+  for (int p0 = 0; p0 < 64; p0 += per) {
+    int cnt[64] = {};
+    for (int l = p0; l < p0 + per && l < 64; ++l) {
+      const int m = l / H, h = l % H;
+      const int lx = m % TX, ly = m / TX;
+      const int word = ((((ly + 1) * (TX + 2) + lx + 1) * KP + h * KH) * EB) / 4;
+      for (int w = 0; w < W / 4; ++w) cnt[(word + w) % NB] += 1;
+    }
+    int mx = 0;
+    for (int b = 0; b < NB; ++b) mx = cnt[b] > mx ? cnt[b] : mx;
+    total += mx - 1;
+  }
+  return total;
+}
+
+// Channel layout of one tile shape.  H threads per cell: H = 1, or H = 2 (even K: the two
+// lanes of a pair (2m, 2m+1) hold channels [0, K/2) and [K/2, K) of the same cell).
+template <class TS, int K, int TX, int TY, int H> struct Tile {
   using TC = typename Traits<TS>::TC;
+  static_assert(H == 1 || (H == 2 && K % 2 == 0), "H = 2 needs even K");
+  static constexpr int KH = K / H;               // channels per thread
   // widest global chunk: 16 bytes of TS and of TC
   static constexpr int VCAP = (16 / (int)sizeof(TC)) < (16 / (int)sizeof(TS)) ? 16 / (int)sizeof(TC)
                                                                               : 16 / (int)sizeof(TS);
@@ -423,58 +448,73 @@ template <class TS, int K> struct Layout {
   static constexpr int VEC = SPLIT ? VCAP : p2div(K, VCAP);
   static constexpr int TAIL = SPLIT ? K % VCAP : 0;
   static constexpr int GA = SPLIT ? (int)sizeof(TS) : VEC * (int)sizeof(TS);
-  // LDS read vector (TC elements) and the cell stride KP: a multiple of LV with KP / LV
-  // odd, so consecutive lanes' LV-wide reads hit distinct banks
-  static constexpr int LV = p2div(K, 16 / (int)sizeof(TC)) >= 2 ? 16 / (int)sizeof(TC) : 16 / (int)sizeof(TC);
-  static constexpr int KP0 = (K + LV - 1) / LV;
-  static constexpr int KP = (KP0 % 2 == 1 ? KP0 : KP0 + 1) * LV;
-  static constexpr int NG = KP0;  // LV-groups read per neighbour (the last may hold padding)
-  static constexpr int WV = p2div(KP, VEC);  // LDS write vector of a load chunk (divides KP and VEC)
-  static constexpr int SVS = p2div(K, 16 / (int)sizeof(TS));  // per-cell store vector (TS elements)
-};
-
-template <class TS, int K, int TX, int TY> struct Tile {
-  using L = Layout<TS, K>;
-  using TC = typename L::TC;
-  static constexpr int NT = TX * TY;             // threads = cells per workgroup
+  // LDS read vector (TC elements): 16 bytes for H = 1 (the last group of a cell may read
+  // padding), else the widest power of two dividing KH; NG groups per neighbour
+  static constexpr int LV = H == 1 ? 16 / (int)sizeof(TC) : p2div(KH, 16 / (int)sizeof(TC));
+  static constexpr int NG = (KH + LV - 1) / LV;
+  // cell stride KP (TC elements): the multiple of LV covering every lane's reads
+  // ((H - 1) * KH + NG * LV >= K), among the next five, with the fewest bank conflicts of the
+  // neighbour reads for 32 and 64 banks
+  static constexpr int KMIN = (H - 1) * KH + NG * LV;
+  static constexpr int pick_kp() {
+    int best = KMIN, bc = 1 << 30;
+    for (int c = 0; c < 5; ++c) {
+      const int kp = KMIN + c * LV;
+      const int n = lds_conflicts(kp, KH, LV, H, TX, (int)sizeof(TC), 32) +
+                    lds_conflicts(kp, KH, LV, H, TX, (int)sizeof(TC), 64);
+      if (n < bc) {
+        bc = n;
+        best = kp;
+      }
+    }
+    return best;
+  }
+  static constexpr int KP = pick_kp();
+  static constexpr int WV = p2div(KP, VEC);                   // LDS write vector of a load chunk
+  static constexpr int SVS = p2div(KH, 16 / (int)sizeof(TS));  // staging store vector (TS elements)
+  static constexpr int NC = TX * TY;             // cells per workgroup
+  static constexpr int NT = NC * H;              // threads per workgroup
   static constexpr int PX = TX + 2, PY = TY + 2;
-  static constexpr int CPC = K / L::VEC;         // full chunks per cell
+  static constexpr int CPC = K / VEC;            // full chunks per cell
   static constexpr int NCH = PY * PX * CPC;      // full chunks per plane
   static constexpr int CPT = (NCH + NT - 1) / NT;
-  static constexpr int NCT = L::TAIL ? PY * PX : 0;          // tail chunks per plane (one per cell)
+  static constexpr int NCT = TAIL ? PY * PX : 0;  // tail chunks per plane (one per cell)
   static constexpr int CPTT = (NCT + NT - 1) / NT;
-  static constexpr int PLANE = PY * PX * L::KP;  // TC elements per LDS plane
-  static constexpr int STAGE = PF_STORE_MODE != 0 ? NT * K : 8;  // TS elements of output staging
+  static constexpr int PLANE = PY * PX * KP;     // TC elements per LDS plane
+  static constexpr int STAGE = PF_STORE_MODE != 0 ? NC * K : 8;  // TS elements of output staging
   static constexpr int AXN = 12 * K;                              // TC elements of the LDS axis table
   static constexpr size_t LDS_BYTES = 2 * PLANE * sizeof(TC) + STAGE * sizeof(TS) + AXN * sizeof(TC);
 };
 
+// The sum of squares S = sum_i eta_i^2 of a cell is formed as two sequential FMA chains, over
+// channels [0, K/2) and [K/2, K), added low + high.  Every kernel uses this order, so a cell's
+// S is bitwise the same whether one thread or a lane pair holds its channels.
+template <class TC>
+__device__ __forceinline__ TC sum_sq_halves(TC lo, TC hi) { return lo + hi; }
+
 // ------------------------------------------------------------------ 2.5-D march of one tile
-// A workgroup of TX*TY threads owns a TX x TY column of cells and marches it through the
-// z-planes [zb, zb + len), upward for even z-chunks and downward for odd ones: two z-adjacent
-// chunks then read their shared boundary planes at the same time (both at their start or both
-// at their end), so the second read is served from L2.  Planes (tile + one-cell halo ring,
-// edge-replicated at the domain faces) are double-buffered in LDS; the plane two ahead is
-// prefetched into registers while the current plane is computed.  Each thread keeps the K
-// channels of its own cell at the planes behind, at and ahead of the current one in three
-// register arrays whose roles rotate by renaming (the z loop is unrolled by three).  The
-// stencil uses behind + ahead (commutative) and ahead - behind as the z difference, whose
-// sign the downward march compensates in the axis table (stage_axes).
+// A workgroup owns a TX x TY column of cells (H threads per cell) and marches it upward through
+// the z-planes [zb, zb + len).  Planes (tile + one-cell halo ring, edge-replicated at the domain
+// faces) are double-buffered in LDS; the plane two ahead is prefetched into registers while the
+// current plane is computed.  Each thread keeps its KH channels of its cell at z-1, z and z+1
+// in three register arrays whose roles rotate by renaming (the z loop is unrolled by three).
+// With H = 2 the two lanes of a pair exchange their partial sums of squares with a lane
+// shuffle.
 // Results are staged in LDS and written back as contiguous row segments (PF_STORE_MODE 1, 2)
 // or stored per cell (0).  The instance and step arguments are read through references (kernel
 // argument or device memory), so uniform values stay in scalar registers.
-template <int K, class TS, bool ANISO, int TX, int TY>
+template <int K, class TS, bool ANISO, int TX, int TY, int H>
 struct March {
   using TC = typename Traits<TS>::TC;
-  using L = Layout<TS, K>;
-  using G = Tile<TS, K, TX, TY>;
-  static constexpr int VEC = L::VEC, KP = L::KP, LV = L::LV, NG = L::NG, WV = L::WV, TAIL = L::TAIL;
+  using G = Tile<TS, K, TX, TY, H>;
+  static constexpr int VEC = G::VEC, KP = G::KP, LV = G::LV, NG = G::NG, WV = G::WV, TAIL = G::TAIL;
+  static constexpr int KH = G::KH, KS = K / 2;
   static constexpr int TV = TAIL > 0 ? TAIL : 1;
-  using SV = VecA<TS, VEC, L::GA>;
-  using TLv = VecA<TS, TV, L::GA>;
+  using SV = VecA<TS, VEC, G::GA>;
+  using TLv = VecA<TS, TV, G::GA>;
   using WVv = Vec<TC, WV>;
   using LVv = Vec<TC, LV>;
-  using SVv = Vec<TS, L::SVS>;
+  using SVv = Vec<TS, G::SVS>;
 
   const StepDev& a;
   const StepDyn& d;
@@ -488,33 +528,36 @@ struct March {
   int tsrc_off[G::CPTT > 0 ? G::CPTT : 1];
   int tdst_off[G::CPTT > 0 ? G::CPTT : 1];
   TLv pret[G::CPTT > 0 ? G::CPTT : 1];
-  int gx, gy, own, oxm, oxp, oym, oyp, zb, len, dir;
+  int gx, gy, cl, ch0, own, oxm, oxp, oym, oyp, zb, len, x0, y0;
   bool active;
 
   __device__ __forceinline__ March(const StepDev& a_, const StepDyn& d_, int tile_x, int tile_y, int tile_z,
                                    TC* smem, TS* stage_, TC* ax_)
       : a(a_), d(d_) {
     const int tid = threadIdx.x;
-    const int lx = tid % TX, ly = tid / TX;
-    gx = tile_x * TX + lx;
-    gy = tile_y * TY + ly;
+    cl = H == 1 ? tid : tid >> 1;
+    ch0 = H == 1 ? 0 : (tid & 1) * KH;
+    const int lx = cl % TX, ly = cl / TX;
+    x0 = tile_x * TX;
+    y0 = tile_y * TY;
+    gx = x0 + lx;
+    gy = y0 + ly;
     active = (gx < a.nx) && (gy < a.ny);
     zb = tile_z * a.zc;
     len = min(zb + a.zc, a.nz) - zb;
-    dir = (tile_z & 1) ? -1 : 1;
     buf0 = smem;
     buf1 = smem + G::PLANE;
     stage = stage_;
     ax = ax_;
-    if constexpr (ANISO) stage_axes<TC, K>((const TC*)a.axes, ax_, tid, G::NT, dir);
+    if constexpr (ANISO) stage_axes<TC, K>((const TC*)a.axes, ax_, tid, G::NT);
 #pragma unroll
     for (int j = 0; j < G::CPT; ++j) {
       const int w = tid + j * G::NT;
       const int ww = w < G::NCH ? w : G::NCH - 1;
       const int cs = ww / G::CPC, v = ww % G::CPC;
       const int r = cs / G::PX, cc = cs % G::PX;
-      const int sy = clampi(tile_y * TY - 1 + r, 0, a.ny - 1);
-      const int sx = clampi(tile_x * TX - 1 + cc, 0, a.nx - 1);
+      const int sy = clampi(y0 - 1 + r, 0, a.ny - 1);
+      const int sx = clampi(x0 - 1 + cc, 0, a.nx - 1);
       src_off[j] = (sy * a.nx + sx) * K + v * VEC;
       dst_off[j] = (w < G::NCH) ? cs * KP + v * VEC : -1;
     }
@@ -524,13 +567,13 @@ struct March {
         const int w = tid + j * G::NT;
         const int cs = w < G::NCT ? w : G::NCT - 1;
         const int r = cs / G::PX, cc = cs % G::PX;
-        const int sy = clampi(tile_y * TY - 1 + r, 0, a.ny - 1);
-        const int sx = clampi(tile_x * TX - 1 + cc, 0, a.nx - 1);
+        const int sy = clampi(y0 - 1 + r, 0, a.ny - 1);
+        const int sx = clampi(x0 - 1 + cc, 0, a.nx - 1);
         tsrc_off[j] = (sy * a.nx + sx) * K + G::CPC * VEC;
         tdst_off[j] = (w < G::NCT) ? cs * KP + G::CPC * VEC : -1;
       }
     }
-    own = ((ly + 1) * G::PX + (lx + 1)) * KP;
+    own = ((ly + 1) * G::PX + (lx + 1)) * KP + ch0;
     oxm = own - KP;
     oxp = own + KP;
     oym = own - G::PX * KP;
@@ -571,35 +614,50 @@ struct March {
       }
     }
   }
-  __device__ __forceinline__ void read_center(const TC* b, TC (&dst)[K]) {
+  __device__ __forceinline__ void read_center(const TC* b, TC (&dst)[KH]) {
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
       const LVv v = *(const LVv*)(b + own + g * LV);
 #pragma unroll
       for (int l = 0; l < LV; ++l)
-        if (g * LV + l < K) dst[g * LV + l] = v.v[l];
+        if (g * LV + l < KH) dst[g * LV + l] = v.v[l];
     }
   }
 
-  // plane of march step n, and a plane index moved by d steps along the march (clamped)
-  __device__ __forceinline__ int zplane(int n) const { return dir > 0 ? zb + n : zb + len - 1 - n; }
-  __device__ __forceinline__ int zstep(int z, int d) const { return clampi(z + d * dir, 0, a.nz - 1); }
+  // S of this thread's cell (order: sum_sq_halves of the two chains [0, KS) and [KS, K))
+  __device__ __forceinline__ TC cell_sum_sq(const TC (&c)[KH]) {
+    if constexpr (H == 1) {
+      TC lo = 0, hi = 0;
+#pragma unroll
+      for (int i = 0; i < KS; ++i) lo = fmac(c[i], c[i], lo);
+#pragma unroll
+      for (int i = KS; i < K; ++i) hi = fmac(c[i], c[i], hi);
+      return sum_sq_halves(lo, hi);
+    } else {
+      TC part = 0;
+#pragma unroll
+      for (int i = 0; i < KH; ++i) part = fmac(c[i], c[i], part);
+      const TC oth = __shfl_xor(part, 1, 64);
+      return ch0 == 0 ? sum_sq_halves(part, oth) : sum_sq_halves(oth, part);
+    }
+  }
 
-  // Compute plane z = zplane(n) from (lo, c, hi) = planes behind, at and ahead of z, then (if another plane
+  // plane z + d clamped to the domain (edge replication in z)
+  __device__ __forceinline__ int zstep(int z, int d) const { return clampi(z + d, 0, a.nz - 1); }
+
+  // Compute plane z = zb + n from (lo, c, hi) = planes z-1, z, z+1, then (if another plane
   // follows) rotate the LDS ring and read plane z+2 into `lo`.
-  __device__ __forceinline__ void body(int n, TC (&lo)[K], TC (&c)[K], TC (&hi)[K]) {
-    const int z = zplane(n);
+  __device__ __forceinline__ void body(int n, TC (&lo)[KH], TC (&c)[KH], TC (&hi)[KH]) {
+    const int z = zb + n;
     TC* cb = (n & 1) ? buf1 : buf0;
     if (active) {
       const long long cell = ((long long)z * a.ny + gy) * a.nx + gx;
-      TC S = 0;
-#pragma unroll
-      for (int i = 0; i < K; ++i) S = fmac(c[i], c[i], S);
+      const TC S = cell_sum_sq(c);
       const CellCoef<TC> cf = cell_coef<TC>(a, d.cold, gx, gy, z, cell, d.t_prev, d.hstep, S);
       const TC B = lap_B<TC>(a);
       TC ihx2, ihy2, ihz2;
       inv_h2<TC>(a, ihx2, ihy2, ihz2);
-      TC res[K];
+      TC res[KH];
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         const LVv vxm = *(const LVv*)(cb + oxm + g * LV);
@@ -609,12 +667,12 @@ struct March {
 #pragma unroll
         for (int l = 0; l < LV; ++l) {
           const int i = g * LV + l;
-          if (i < K) {
+          if (i < KH) {
             // A channel that is exactly zero on the whole stencil of every lane stays
             // exactly +0 (the full update evaluates to +0 there); skip it wave-wide.
             if (__ballot(any_nz(c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l], lo[i], hi[i])))
               res[i] = update_one<TC, ANISO>(cf, B, ihx2, ihy2, ihz2, c[i], vxm.v[l], vxp.v[l], vym.v[l],
-                                             vyp.v[l], lo[i], hi[i], ax + 12 * i);
+                                             vyp.v[l], lo[i], hi[i], ax + 12 * (ch0 + i));
             else
               res[i] = (TC)0;
           }
@@ -622,19 +680,19 @@ struct March {
       }
       if (__ballot(cf.liquid)) {
 #pragma unroll
-        for (int i = 0; i < K; ++i) res[i] = cf.liquid ? (TC)0 : res[i];
+        for (int i = 0; i < KH; ++i) res[i] = cf.liquid ? (TC)0 : res[i];
       }
 #if PF_STORE_MODE == 0
-      TS* o = (TS*)d.out + cell * K;
+      TS* o = (TS*)d.out + cell * K + ch0;
 #else
-      TS* o = stage + threadIdx.x * K;
+      TS* o = stage + cl * K + ch0;
 #endif
 #pragma unroll
-      for (int j = 0; j < K / L::SVS; ++j) {
+      for (int j = 0; j < KH / G::SVS; ++j) {
         SVv s;
 #pragma unroll
-        for (int e = 0; e < L::SVS; ++e) s.v[e] = from_tc<TS>(res[j * L::SVS + e]);
-        *(SVv*)(o + j * L::SVS) = s;
+        for (int e = 0; e < G::SVS; ++e) s.v[e] = from_tc<TS>(res[j * G::SVS + e]);
+        *(SVv*)(o + j * G::SVS) = s;
       }
     }
     lds_barrier();
@@ -666,8 +724,6 @@ struct March {
     using UT = typename RawT<U>::type;
     constexpr int CPR = ROWB / CH;
     constexpr int NCHO = TY * CPR;
-    const int x0 = gx - (int)(threadIdx.x % TX);
-    const int y0 = gy - (int)(threadIdx.x / TX);
     const int rowb = min(TX, a.nx - x0) * CELLB;
     TS* out = (TS*)d.out;
 #pragma unroll
@@ -695,10 +751,9 @@ struct March {
   }
 
   __device__ __forceinline__ void run() {
-    TC r0[K], r1[K], r2[K];
-    // prologue: the first two planes into LDS; the plane behind the first (edge-replicated)
-    // straight into registers
-    const int z0 = zplane(0);
+    TC r0[KH], r1[KH], r2[KH];
+    // prologue: planes zb, zb+1 into LDS; plane zb-1 (edge-replicated) straight into registers
+    const int z0 = zb;
     load_plane(z0);
     store_plane(buf0);
     load_plane(zstep(z0, 1));
@@ -706,18 +761,17 @@ struct March {
     {
       const int zbh = zstep(z0, -1);
       const int sxo = min(gx, a.nx - 1), syo = min(gy, a.ny - 1);
-      const TS* p = (const TS*)d.in + (long long)zbh * plane_elems() + (long long)(syo * a.nx + sxo) * K;
+      const TS* p =
+          (const TS*)d.in + (long long)zbh * plane_elems() + (long long)(syo * a.nx + sxo) * K + ch0;
+      using HV = VecA<TS, VEC, (int)sizeof(TS)>;
 #pragma unroll
-      for (int j = 0; j < G::CPC; ++j) {
-        const SV s = *(const SV*)(p + j * VEC);
+      for (int j = 0; j < KH / VEC; ++j) {
+        const HV s = *(const HV*)(p + j * VEC);
 #pragma unroll
         for (int e = 0; e < VEC; ++e) r0[j * VEC + e] = to_tc(s.v[e]);
       }
-      if constexpr (TAIL > 0) {
-        const TLv s = *(const TLv*)(p + G::CPC * VEC);
 #pragma unroll
-        for (int e = 0; e < TAIL; ++e) r0[G::CPC * VEC + e] = to_tc(s.v[e]);
-      }
+      for (int e = (KH / VEC) * VEC; e < KH; ++e) r0[e] = to_tc(p[e]);
     }
     lds_barrier();
     read_center(buf0, r1);
@@ -731,11 +785,11 @@ struct March {
   }
 };
 
-template <int K, class TS, bool ANISO, int TX, int TY>
+template <int K, class TS, bool ANISO, int TX, int TY, int H>
 __device__ __forceinline__ void march_tile(const StepDev& a, const StepDyn& d, int tile_x, int tile_y,
                                            int tile_z, typename Traits<TS>::TC* __restrict__ smem,
                                            TS* __restrict__ stage, typename Traits<TS>::TC* __restrict__ ax) {
-  March<K, TS, ANISO, TX, TY> m(a, d, tile_x, tile_y, tile_z, smem, stage, ax);
+  March<K, TS, ANISO, TX, TY, H> m(a, d, tile_x, tile_y, tile_z, smem, stage, ax);
   m.run();
 }
 
@@ -757,11 +811,16 @@ __device__ __forceinline__ void generic_cell(const StepDev& a, const StepDyn& dy
   const long long oxm = ix > 0 ? -sx : 0, oxp = ix < nx - 1 ? sx : 0;
   const long long oym = iy > 0 ? -sy : 0, oyp = iy < ny - 1 ? sy : 0;
   const long long ozm = iz > 0 ? -sz : 0, ozp = iz < nz - 1 ? sz : 0;
-  TC S = 0;
-  for (int i = 0; i < K; ++i) {
+  TC slo = 0, shi = 0;  // the two chains of sum_sq_halves
+  for (int i = 0; i < K / 2; ++i) {
     const TC c = to_tc(in[base + i]);
-    S = fmac(c, c, S);
+    slo = fmac(c, c, slo);
   }
+  for (int i = K / 2; i < K; ++i) {
+    const TC c = to_tc(in[base + i]);
+    shi = fmac(c, c, shi);
+  }
+  const TC S = sum_sq_halves(slo, shi);
   const CellCoef<TC> cf = cell_coef<TC>(a, dy.cold, ix, iy, iz, cell, dy.t_prev, dy.hstep, S);
   const TC B = lap_B<TC>(a);
   TC ihx2, ihy2, ihz2;
@@ -788,14 +847,15 @@ __device__ __forceinline__ void generic_cell(const StepDev& a, const StepDyn& dy
   }
 }
 
-// XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so give each
-// XCD a contiguous range of logical tiles (z-chunk fastest, then x, then y) to keep the
-// halo reuse of neighbouring tiles, in z as well as in x and y, inside one L2.
-__device__ __forceinline__ int xcd_swizzle(int b, int n) {
-  constexpr int NX = 8;
-  const int per = n / NX, rem = n % NX;
-  const int x = b % NX, l = b / NX;
-  return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
+// Tile order of a launch: x fastest, then y, then z-chunks from the top of the domain down.
+// Workgroups are dispatched in index order (dealt round-robin to the XCDs), so the top layer,
+// where a surface heat source makes cells hot and the per-cell FP64 path slow, starts first
+// and the cheaper layers below fill in behind it; every XCD gets a share of each layer.
+__device__ __forceinline__ void tile_coords(int t, int tiles_x, int tiles_y, int tiles_z, int& tx, int& ty,
+                                            int& tz) {
+  tx = t % tiles_x;
+  ty = (t / tiles_x) % tiles_y;
+  tz = tiles_z - 1 - t / (tiles_x * tiles_y);
 }
 
 }  // namespace pf

@@ -31,21 +31,17 @@ __host__ __device__ inline StepDyn slot_dyn(const StepDev& a, const BatchSlot& s
   return make_dyn(a, s.eta[s.parity], s.eta[s.parity ^ 1], tp, tc - tp);
 }
 
-template <int K, class TS, bool ANISO, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) void step_kernel(const StepDev a,
-                                                                                           const StepDyn d) {
+template <int K, class TS, bool ANISO, int TX, int TY, int H>
+__global__ __launch_bounds__(TX* TY* H) __attribute__((amdgpu_waves_per_eu(H + 1))) void step_kernel(const StepDev a,
+                                                                                               const StepDyn d) {
   using TC = typename Traits<TS>::TC;
-  using G = Tile<TS, K, TX, TY>;
+  using G = Tile<TS, K, TX, TY, H>;
   __shared__ __attribute__((aligned(16))) TC smem[2 * G::PLANE];
   __shared__ __attribute__((aligned(16))) TS stage[G::STAGE];
   __shared__ __attribute__((aligned(16))) TC ax[ANISO ? G::AXN : 4];
-  const int n = a.tiles_x * a.tiles_y * a.tiles_z;
-  const int t = xcd_swizzle(blockIdx.x, n);
-  // z fastest: the z-chunks of one column run together on one XCD
-  const int tz = t % a.tiles_z;
-  const int tx = (t / a.tiles_z) % a.tiles_x;
-  const int ty = t / (a.tiles_z * a.tiles_x);
-  march_tile<K, TS, ANISO, TX, TY>(a, d, tx, ty, tz, smem, stage, ax);
+  int tx, ty, tz;
+  tile_coords(blockIdx.x, a.tiles_x, a.tiles_y, a.tiles_z, tx, ty, tz);
+  march_tile<K, TS, ANISO, TX, TY, H>(a, d, tx, ty, tz, smem, stage, ax);
 }
 
 template <class TS, bool ANISO, bool RHS = false>
@@ -59,12 +55,12 @@ __global__ __launch_bounds__(256) void generic_kernel(const StepDev a, const Ste
 // tile) of one (dtype, K, anisotropy, tile shape) class.  Every workgroup exits after
 // the list is exhausted; slots with no steps left are skipped.  descs[slot] and
 // dyns[slot] are read in place (scalar loads), never copied into registers wholesale.
-template <int K, class TS, bool ANISO, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) void batch_kernel(
+template <int K, class TS, bool ANISO, int TX, int TY, int H>
+__global__ __launch_bounds__(TX* TY* H) __attribute__((amdgpu_waves_per_eu(2))) void batch_kernel(
     const StepDev* __restrict__ descs, const StepDyn* __restrict__ dyns, const BatchSlot* __restrict__ slots,
     const WorkItem* __restrict__ work_all, const int2* __restrict__ cinfo, int cls) {
   using TC = typename Traits<TS>::TC;
-  using G = Tile<TS, K, TX, TY>;
+  using G = Tile<TS, K, TX, TY, H>;
   __shared__ __attribute__((aligned(16))) TC smem[2 * G::PLANE];
   __shared__ __attribute__((aligned(16))) TS stage[G::STAGE];
   __shared__ __attribute__((aligned(16))) TC ax[ANISO ? G::AXN : 4];
@@ -75,10 +71,9 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(2))) voi
     const BatchSlot& s = slots[it.slot];
     if (s.steps_left <= 0 || !s.active) continue;
     const StepDev& a = descs[it.slot];
-    const int tz = it.tile % a.tiles_z;
-    const int tx = (it.tile / a.tiles_z) % a.tiles_x;
-    const int ty = it.tile / (a.tiles_z * a.tiles_x);
-    march_tile<K, TS, ANISO, TX, TY>(a, dyns[it.slot], tx, ty, tz, smem, stage, ax);
+    int tx, ty, tz;
+    tile_coords(it.tile, a.tiles_x, a.tiles_y, a.tiles_z, tx, ty, tz);
+    march_tile<K, TS, ANISO, TX, TY, H>(a, dyns[it.slot], tx, ty, tz, smem, stage, ax);
     lds_barrier();  // LDS is reused by the next tile
   }
 }
@@ -108,23 +103,23 @@ using BatchLaunch = hipError_t (*)(const StepDev*, const StepDyn*, const BatchSl
                                    int cls, int grid, hipStream_t);
 
 struct StepEntry {
-  int dtype, k, aniso, tile, tx, ty;
+  int dtype, k, aniso, tile, tx, ty, h;
   StepLaunch step;
   BatchLaunch batch;
   size_t lds;
 };
 
-template <int K, class TS, bool ANISO, int TX, int TY>
+template <int K, class TS, bool ANISO, int TX, int TY, int H>
 hipError_t launch_step(const StepDev& a, const StepDyn& d, hipStream_t s) {
   const int n = a.tiles_x * a.tiles_y * a.tiles_z;
-  step_kernel<K, TS, ANISO, TX, TY><<<dim3(n), dim3(TX * TY), 0, s>>>(a, d);
+  step_kernel<K, TS, ANISO, TX, TY, H><<<dim3(n), dim3(TX * TY * H), 0, s>>>(a, d);
   return hipGetLastError();
 }
 
-template <int K, class TS, bool ANISO, int TX, int TY>
+template <int K, class TS, bool ANISO, int TX, int TY, int H>
 hipError_t launch_batch(const StepDev* a, const StepDyn* d, const BatchSlot* sl, const WorkItem* w, const int2* ci,
                         int cls, int grid, hipStream_t s) {
-  batch_kernel<K, TS, ANISO, TX, TY><<<dim3(grid), dim3(TX * TY), 0, s>>>(a, d, sl, w, ci, cls);
+  batch_kernel<K, TS, ANISO, TX, TY, H><<<dim3(grid), dim3(TX * TY * H), 0, s>>>(a, d, sl, w, ci, cls);
   return hipGetLastError();
 }
 
@@ -142,9 +137,17 @@ hipError_t launch_batch_generic(const StepDev* a, const StepDyn* d, const BatchS
   return hipGetLastError();
 }
 
-#define PF_ENTRY(DT, TS, K, A, T, TX, TY)                                                             \
-  {DT, K, A, T, TX, TY, &launch_step<K, TS, (A != 0), TX, TY>, &launch_batch<K, TS, (A != 0), TX, TY>, \
-   Tile<TS, K, TX, TY>::LDS_BYTES}
+#define PF_ENTRY(DT, TS, K, A, T, TX, TY, H)                                      \
+  {DT,                                                                           \
+   K,                                                                            \
+   A,                                                                            \
+   T,                                                                            \
+   TX,                                                                           \
+   TY,                                                                           \
+   H,                                                                            \
+   &launch_step<K, TS, (A != 0), TX, TY, H>,                                     \
+   &launch_batch<K, TS, (A != 0), TX, TY, H>,                                    \
+   Tile<TS, K, TX, TY, H>::LDS_BYTES}
 
 // instantiation tables, one per compiled part of pf_inst.hip (see Makefile):
 // parts 0-3 F32, 4-7 BF16, 8-11 F64, each over a K range

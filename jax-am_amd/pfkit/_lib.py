@@ -61,6 +61,7 @@ SIGNATURES = {
     "pf_rhs": (ctypes.c_int, [_VP, _VP, _I32, _VP, ctypes.POINTER(ParamsT), _VP, _D, _VP]),
     "pf_tile_count": (ctypes.c_int, [_I32, _I32]),
     "pf_tile_shape": (ctypes.c_int, [_I32, _I32, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
+    "pf_tile_lanes": (ctypes.c_int, [_I32, _I32, _I32]),
     "pf_fast_path": (ctypes.c_int, [_I32, _I32]),
     "pf_temperature": (ctypes.c_int, [_VP, _I32, ctypes.POINTER(ParamsT), _D, _VP]),
     "pf_init_onehot": (ctypes.c_int, [_VP, _I32, _VP, _I64, _I32, _VP]),

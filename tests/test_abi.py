@@ -41,11 +41,15 @@ def test_tile_tables():
     lib = _lib.load()
     for dt in (_lib.PF_F32, _lib.PF_BF16):
         for k in list(range(1, 25)) + [32]:
-            assert lib.pf_fast_path(dt, k) == 1 and lib.pf_tile_count(dt, k) == 3
+            # three one-lane shapes; even K also two lane-pair shapes
+            assert lib.pf_fast_path(dt, k) == 1 and lib.pf_tile_count(dt, k) == (5 if k % 2 == 0 else 3)
         tx, ty = ctypes.c_int32(), ctypes.c_int32()
         assert lib.pf_tile_shape(dt, 20, 0, ctypes.byref(tx), ctypes.byref(ty)) == 0
-        assert (tx.value, ty.value) == (32, 8)
-    assert lib.pf_tile_count(_lib.PF_F64, 20) == 2
+        assert (tx.value, ty.value) == (32, 8) and lib.pf_tile_lanes(dt, 20, 0) == 1
+        assert lib.pf_tile_shape(dt, 20, 3, ctypes.byref(tx), ctypes.byref(ty)) == 0
+        assert (tx.value, ty.value) == (16, 8) and lib.pf_tile_lanes(dt, 20, 3) == 2
+        assert lib.pf_tile_lanes(dt, 19, 3) < 0
+    assert lib.pf_tile_count(_lib.PF_F64, 20) == 4 and lib.pf_tile_count(_lib.PF_F64, 19) == 2
     assert lib.pf_fast_path(_lib.PF_F32, 25) == 0 and lib.pf_fast_path(_lib.PF_F32, 64) == 0
     assert lib.pf_tile_shape(_lib.PF_F32, 20, 7, None, None) != 0
 
