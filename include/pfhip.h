@@ -221,6 +221,26 @@ int pf_batch_run(pf_batch_t* b, int32_t steps, int32_t chunk);
 int pf_batch_status(pf_batch_t* b, int32_t* steps_left, double* t, int32_t* parity, int32_t n);
 int pf_batch_active(const pf_batch_t* b);
 
+/*
+ * Grain segmentation (jax_am/phase_field/utils.py:181-265, process_eta / BFS): connected
+ * components of face-adjacent non-liquid cells (T < T_liquidus at time t, temperature from
+ * p->laser or T_dev as in pf_step) with equal orientation index ori_dev[N] (argmax eta).
+ * On return labels_dev[N] holds, per cell, the smallest cell index of its grain (-1 for liquid
+ * cells); *n_grains grains exist; grain g (0 <= g < *n_grains, in no particular order) has root
+ * cell root_dev[g] (capacity N) and cid_dev[root_dev[g]] == g (cid_dev: N entries).
+ * ws_dev: 2 int32 of device scratch.  Label propagation passes run until one changes nothing
+ * (at most max_passes, else PF_E_STATE).  Synchronizes the stream.
+ */
+int pf_segment(const int32_t* ori_dev, const void* T_dev, const pf_params_t* p, double t, int32_t* labels_dev,
+               int32_t* cid_dev, int32_t* root_dev, int32_t* ws_dev, int64_t max_passes, int32_t* n_grains,
+               void* stream);
+/* Exact integer moments of each grain of pf_segment into moments_dev[n_grains][10]: number of
+ * cells and the sums of ix, iy, iz, ix^2, iy^2, iz^2, ix*iy, ix*iz, iy*iz over its cells (cell
+ * indices; centroid = (index + 1/2) * h).  utils.py:268-310 derives volumes and aspect ratios
+ * from these. */
+int pf_segment_moments(const int32_t* labels_dev, const int32_t* cid_dev, const pf_params_t* p, int32_t n_grains,
+                       int64_t* moments_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -184,6 +184,34 @@ __host__ __device__ inline StepDyn make_dyn(const StepDev& a, const void* in, vo
 }
 
 // ------------------------------------------------------------------ per-cell physics
+// The Gaussian spot factorises: T = T_ambient + dT_peak * (exy * ez) with
+// exy = exp(-(X^2 + Y^2) / (2 sigma_xy^2)) of the cell column and ez = exp(-Z^2 / (2 sigma_z^2))
+// of the plane.  Every kernel evaluates it in this form, so the march can take exy once per
+// column and ez once per plane and still agree bitwise with the per-cell evaluation.
+__device__ __forceinline__ double gauss_exy64(const StepDev& a, int ix, int iy, double t) {
+  const double vt = a.vel * t;
+  const double X = (((double)ix + 0.5) * a.hx - a.p0x) - vt * a.dirx;
+  const double Y = (((double)iy + 0.5) * a.hy - a.p0y) - vt * a.diry;
+  return exp(-((X * X + Y * Y) * a.g_sxy));
+}
+__device__ __forceinline__ double gauss_ez64(const StepDev& a, int iz, double t) {
+  const double Z = (((double)iz + 0.5) * a.hz - a.p0z) - a.vel * t * a.dirz;
+  return exp(-(Z * Z * a.g_sz));
+}
+__device__ __forceinline__ double gauss_T64(const StepDev& a, double exy, double ez) {
+  return a.T_amb + a.dT_peak * (exy * ez);
+}
+__device__ __forceinline__ float gauss_exy32(const ColdPath& k, int ix, int iy) {
+  const float X = ((float)ix + 0.5f) * k.hx - k.px;
+  const float Y = ((float)iy + 0.5f) * k.hy - k.py;
+  return expf(-((X * X + Y * Y) * k.gxy));
+}
+__device__ __forceinline__ float gauss_ez32(const ColdPath& k, int iz) {
+  const float Z = ((float)iz + 0.5f) * k.hz - k.pz;
+  return expf(-(Z * Z * k.gz));
+}
+__device__ __forceinline__ float gauss_T32(const ColdPath& k, float exy, float ez) { return k.T_amb + k.dT * (exy * ez); }
+
 // Temperature at the centroid of cell (ix, iy, iz) at time t (FP64); cell index c for the buffer mode.
 __device__ __forceinline__ double cell_temperature(const StepDev& a, int ix, int iy, int iz, long long c,
                                                    double t) {
@@ -210,7 +238,10 @@ __device__ __forceinline__ double cell_temperature(const StepDev& a, int ix, int
     return T;
   }
   // Gaussian moving spot
-  return a.T_amb + a.dT_peak * exp(-((X * X + Y * Y) * a.g_sxy + Z * Z * a.g_sz));
+  (void)X;
+  (void)Y;
+  (void)Z;
+  return gauss_T64(a, gauss_exy64(a, ix, iy, t), gauss_ez64(a, iz, t));
 }
 
 // The same temperature in FP32 (cold-path test and value).
@@ -229,7 +260,7 @@ __device__ __forceinline__ float cell_temperature_f32(const StepDev& a, const Co
     if (k.T_cap > 0.0f && T > k.T_cap) T = k.T_cap;
     return T;
   }
-  return k.T_amb + k.dT * expf(-((X * X + Y * Y) * k.gxy + Z * Z * k.gz));
+  return gauss_T32(k, gauss_exy32(k, ix, iy), gauss_ez32(k, iz));
 }
 
 template <class TC> struct CellCoef {
@@ -268,19 +299,20 @@ __device__ __forceinline__ CellCoef<TC> cell_coef64(const StepDev& a, double T, 
   return r;
 }
 
-// Per-cell coefficients.  F64 storage: always the FP64 evaluation.  FP32 arithmetic: each
-// cold-path cell (make_cold) takes an FP32 mobility (zeta = 1, no liquid reset: exact
-// there); every other cell takes the FP64 evaluation.  The choice is
-// made per cell from the cell's own temperature, so results do not depend on which cells
-// share a wavefront; the FP64 branch is entered only by wavefronts holding a hot cell.
-template <class TC>
-__device__ __forceinline__ CellCoef<TC> cell_coef(const StepDev& a, const ColdPath& k, int ix, int iy, int iz,
-                                                  long long c, double t, double hstep, TC S) {
+// Per-cell coefficients from the cell's FP32 temperature T32 and a callable giving its FP64
+// temperature.  F64 storage: always the FP64 evaluation.  FP32 arithmetic: each cold-path cell
+// (make_cold) takes an FP32 mobility (zeta = 1, no liquid reset: exact there); every other cell
+// takes the FP64 evaluation.  The choice is made per cell from the cell's own temperature, so
+// results do not depend on which cells share a wavefront; the FP64 temperature is evaluated,
+// and the FP64 branch entered, only by wavefronts holding a hot cell.
+template <class TC, class F64>
+__device__ __forceinline__ CellCoef<TC> cell_coef_with(const StepDev& a, const ColdPath& k, float T32,
+                                                       const F64& t64, double hstep, TC S) {
   if constexpr (sizeof(TC) == 8) {
     (void)k;
-    return cell_coef64<TC>(a, cell_temperature(a, ix, iy, iz, c, t), S, hstep);
+    (void)T32;
+    return cell_coef64<TC>(a, t64(), S, hstep);
   } else {
-    const float T32 = cell_temperature_f32(a, k, ix, iy, iz, c);
     CellCoef<float> r;
     const float Lg = k.L0 * expf(-k.Qg_R / T32);
     const bool hot = !(T32 < k.T_lim && fabsf(Lg) * k.hs < 1e-6f);  // NaN -> FP64 path
@@ -291,10 +323,18 @@ __device__ __forceinline__ CellCoef<TC> cell_coef(const StepDev& a, const ColdPa
     r.A = fmac(k.gamma4, S, -1.0f);
     r.liquid = false;
     if (__ballot(hot)) {
-      if (hot) r = cell_coef64<float>(a, cell_temperature(a, ix, iy, iz, c, t), S, hstep);
+      if (hot) r = cell_coef64<float>(a, t64(), S, hstep);
     }
     return r;
   }
+}
+
+// Per-cell coefficients of cell (ix, iy, iz) (index c) at time t.
+template <class TC>
+__device__ __forceinline__ CellCoef<TC> cell_coef(const StepDev& a, const ColdPath& k, int ix, int iy, int iz,
+                                                  long long c, double t, double hstep, TC S) {
+  const float T32 = sizeof(TC) == 8 ? 0.0f : cell_temperature_f32(a, k, ix, iy, iz, c);
+  return cell_coef_with<TC>(a, k, T32, [&] { return cell_temperature(a, ix, iy, iz, c, t); }, hstep, S);
 }
 
 // Bitwise "non-zero" test of a channel's 7-point stencil (a -0.0 counts as non-zero).
@@ -483,7 +523,7 @@ template <class TS, int K, int TX, int TY, int H> struct Tile {
   static constexpr int PLANE = PY * PX * KP;     // TC elements per LDS plane
   static constexpr int STAGE = PF_STORE_MODE != 0 ? NC * K : 8;  // TS elements of output staging
   static constexpr int AXN = 12 * K;                              // TC elements of the LDS axis table
-  static constexpr size_t LDS_BYTES = 2 * PLANE * sizeof(TC) + STAGE * sizeof(TS) + AXN * sizeof(TC);
+  static constexpr size_t LDS_BYTES = 2 * PLANE * sizeof(TC) + STAGE * sizeof(TS) + AXN * sizeof(TC) + 768;
 };
 
 // The sum of squares S = sum_i eta_i^2 of a cell is formed as two sequential FMA chains, over
@@ -492,6 +532,13 @@ template <class TS, int K, int TX, int TY, int H> struct Tile {
 template <class TC>
 __device__ __forceinline__ TC sum_sq_halves(TC lo, TC hi) { return lo + hi; }
 
+// Gaussian z-factors of the planes of one march (LDS), for marches of up to kZTab planes.
+constexpr int kZTab = 64;
+struct ZTab {
+  double e64[kZTab];
+  float e32[kZTab];
+};
+
 // ------------------------------------------------------------------ 2.5-D march of one tile
 // A workgroup owns a TX x TY column of cells (H threads per cell) and marches it upward through
 // the z-planes [zb, zb + len).  Planes (tile + one-cell halo ring, edge-replicated at the domain
@@ -499,7 +546,8 @@ __device__ __forceinline__ TC sum_sq_halves(TC lo, TC hi) { return lo + hi; }
 // current plane is computed.  Each thread keeps its KH channels of its cell at z-1, z and z+1
 // in three register arrays whose roles rotate by renaming (the z loop is unrolled by three).
 // With H = 2 the two lanes of a pair exchange their partial sums of squares with a lane
-// shuffle.
+// shuffle.  With the Gaussian source each thread takes its column factor once and each plane's
+// factor comes from an LDS table filled at the start (gauss_exy*, gauss_ez*).
 // Results are staged in LDS and written back as contiguous row segments (PF_STORE_MODE 1, 2)
 // or stored per cell (0).  The instance and step arguments are read through references (kernel
 // argument or device memory), so uniform values stay in scalar registers.
@@ -522,6 +570,7 @@ struct March {
   TC* __restrict__ buf1;
   TS* __restrict__ stage;
   const TC* __restrict__ ax;  // LDS axis table (ANISO)
+  const ZTab* __restrict__ zt;  // LDS Gaussian z-factors of this march's planes (gsep)
   int src_off[G::CPT];
   int dst_off[G::CPT];
   SV pre[G::CPT];
@@ -529,10 +578,12 @@ struct March {
   int tdst_off[G::CPTT > 0 ? G::CPTT : 1];
   TLv pret[G::CPTT > 0 ? G::CPTT : 1];
   int gx, gy, cl, ch0, own, oxm, oxp, oym, oyp, zb, len, x0, y0;
-  bool active;
+  bool active, gsep;
+  double exy64;
+  float exy32;
 
   __device__ __forceinline__ March(const StepDev& a_, const StepDyn& d_, int tile_x, int tile_y, int tile_z,
-                                   TC* smem, TS* stage_, TC* ax_)
+                                   TC* smem, TS* stage_, TC* ax_, ZTab* zt_)
       : a(a_), d(d_) {
     const int tid = threadIdx.x;
     cl = H == 1 ? tid : tid >> 1;
@@ -550,6 +601,18 @@ struct March {
     stage = stage_;
     ax = ax_;
     if constexpr (ANISO) stage_axes<TC, K>((const TC*)a.axes, ax_, tid, G::NT);
+    zt = zt_;
+    gsep = a.tmodel == PF_T_GAUSSIAN && len <= kZTab;
+    exy64 = 0.0;
+    exy32 = 0.0f;
+    if (gsep) {
+      exy64 = gauss_exy64(a, gx, gy, d.t_prev);
+      exy32 = gauss_exy32(d.cold, gx, gy);
+      for (int w = tid; w < len; w += G::NT) {
+        zt_->e64[w] = gauss_ez64(a, zb + w, d.t_prev);
+        zt_->e32[w] = gauss_ez32(d.cold, zb + w);
+      }
+    }
 #pragma unroll
     for (int j = 0; j < G::CPT; ++j) {
       const int w = tid + j * G::NT;
@@ -653,7 +716,13 @@ struct March {
     if (active) {
       const long long cell = ((long long)z * a.ny + gy) * a.nx + gx;
       const TC S = cell_sum_sq(c);
-      const CellCoef<TC> cf = cell_coef<TC>(a, d.cold, gx, gy, z, cell, d.t_prev, d.hstep, S);
+      float T32 = 0.0f;
+      if constexpr (sizeof(TC) == 4)
+        T32 = gsep ? gauss_T32(d.cold, exy32, zt->e32[n]) : cell_temperature_f32(a, d.cold, gx, gy, z, cell);
+      const CellCoef<TC> cf = cell_coef_with<TC>(
+          a, d.cold, T32,
+          [&] { return gsep ? gauss_T64(a, exy64, zt->e64[n]) : cell_temperature(a, gx, gy, z, cell, d.t_prev); },
+          d.hstep, S);
       const TC B = lap_B<TC>(a);
       TC ihx2, ihy2, ihz2;
       inv_h2<TC>(a, ihx2, ihy2, ihz2);
@@ -788,8 +857,9 @@ struct March {
 template <int K, class TS, bool ANISO, int TX, int TY, int H>
 __device__ __forceinline__ void march_tile(const StepDev& a, const StepDyn& d, int tile_x, int tile_y,
                                            int tile_z, typename Traits<TS>::TC* __restrict__ smem,
-                                           TS* __restrict__ stage, typename Traits<TS>::TC* __restrict__ ax) {
-  March<K, TS, ANISO, TX, TY, H> m(a, d, tile_x, tile_y, tile_z, smem, stage, ax);
+                                           TS* __restrict__ stage, typename Traits<TS>::TC* __restrict__ ax,
+                                           ZTab* __restrict__ zt) {
+  March<K, TS, ANISO, TX, TY, H> m(a, d, tile_x, tile_y, tile_z, smem, stage, ax, zt);
   m.run();
 }
 

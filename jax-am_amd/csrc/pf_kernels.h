@@ -39,9 +39,10 @@ __global__ __launch_bounds__(TX* TY* H) __attribute__((amdgpu_waves_per_eu(H + 1
   __shared__ __attribute__((aligned(16))) TC smem[2 * G::PLANE];
   __shared__ __attribute__((aligned(16))) TS stage[G::STAGE];
   __shared__ __attribute__((aligned(16))) TC ax[ANISO ? G::AXN : 4];
+  __shared__ ZTab zt;
   int tx, ty, tz;
   tile_coords(blockIdx.x, a.tiles_x, a.tiles_y, a.tiles_z, tx, ty, tz);
-  march_tile<K, TS, ANISO, TX, TY, H>(a, d, tx, ty, tz, smem, stage, ax);
+  march_tile<K, TS, ANISO, TX, TY, H>(a, d, tx, ty, tz, smem, stage, ax, &zt);
 }
 
 template <class TS, bool ANISO, bool RHS = false>
@@ -64,6 +65,7 @@ __global__ __launch_bounds__(TX* TY* H) __attribute__((amdgpu_waves_per_eu(2))) 
   __shared__ __attribute__((aligned(16))) TC smem[2 * G::PLANE];
   __shared__ __attribute__((aligned(16))) TS stage[G::STAGE];
   __shared__ __attribute__((aligned(16))) TC ax[ANISO ? G::AXN : 4];
+  __shared__ ZTab zt;
   const int2 ci = cinfo[cls];  // (offset, count) of this class's work list
   const WorkItem* __restrict__ work = work_all + ci.x;
   for (int w = blockIdx.x; w < ci.y; w += gridDim.x) {
@@ -73,7 +75,7 @@ __global__ __launch_bounds__(TX* TY* H) __attribute__((amdgpu_waves_per_eu(2))) 
     const StepDev& a = descs[it.slot];
     int tx, ty, tz;
     tile_coords(it.tile, a.tiles_x, a.tiles_y, a.tiles_z, tx, ty, tz);
-    march_tile<K, TS, ANISO, TX, TY, H>(a, dyns[it.slot], tx, ty, tz, smem, stage, ax);
+    march_tile<K, TS, ANISO, TX, TY, H>(a, dyns[it.slot], tx, ty, tz, smem, stage, ax, &zt);
     lds_barrier();  // LDS is reused by the next tile
   }
 }

@@ -196,6 +196,21 @@ class PFSolver:
             a, b = b, a
         return a
 
+    def run_toolpath(self, eta0, toolpath, steps=None, callback=None):
+        """Advance along a ``pfkit.laser.Toolpath`` (multi_scan/example.py:88-105): step i goes from
+        ts[i] to ts[i+1] with the source of ``toolpath.laser(i)``.  ``callback(i + 1, eta)`` runs after
+        each step (eta is the live buffer; copy it to keep it).  Returns the final eta."""
+        ts = toolpath.schedule()[0]
+        n = len(ts) - 1 if steps is None else min(int(steps), len(ts) - 1)
+        a = self.state_rhs.check_state(eta0).clone()
+        b = torch.empty_like(a)
+        for i in range(n):
+            self.state_rhs.step(a, b, toolpath.laser(i), ts[i], ts[i + 1], self.tile, self.zchunk)
+            a, b = b, a
+            if callback is not None:
+                callback(i + 1, a)
+        return a
+
     def ini_cond(self):
         """allen_cahn.py:145-153: one-hot eta[c, cell_ori_inds[c]] = 1."""
         eta = torch.empty((self.N, self.K), dtype=self.dtype, device=self.device)

@@ -96,3 +96,79 @@ class Quench:
 
 
 LASERS = (Rosenthal, Gaussian, Quench)
+
+
+@dataclass
+class Toolpath:
+    """Multi-segment scan path, applications/phase_field/multi_scan/example.py:52-84 (read_path)
+    driving the Rosenthal source of :33-49 (Q = 25 W, alpha = 5.2e-6, kappa = 27, T capped at 2000).
+
+    Segment i runs from corner i to corner i+1 at speed ``vel``: its step times are
+    ``numpy.arange(t_pre, t_pre + |d_i| / vel, dt)`` and its source positions are
+    ``numpy.linspace(corner_i, corner_{i+1}, n_i)`` (so the last position of a segment is the
+    next corner), with the segment's unit direction.  ``schedule()`` returns those arrays;
+    ``laser(i)`` is the Rosenthal descriptor of step i: it places the source at the i-th position
+    at time ts[i], so it works with ``PFSolver.stepper`` (T is evaluated at t_prev = ts[i]) and
+    with ``PFSolver.run_toolpath``.  The reference reads a per-segment power switch but does not
+    use it in T; ``apply_switch=True`` turns the source off (T = T_ambient) on switched-off segments.
+    """
+    x_pos: Tuple[float, ...]
+    y_pos: Tuple[float, ...]
+    z_pos: Tuple[float, ...]
+    switch: Tuple[float, ...]
+    vel: float
+    dt: float
+    Q: float = 25.0
+    alpha: float = 5.2e-6
+    kappa_T: float = 27.0
+    T_ambient: float = 300.0
+    T_cap: float = 2000.0
+    apply_switch: bool = False
+
+    @classmethod
+    def from_args(cls, pf_args, **kw):
+        lp = pf_args["laser_path"]
+        return cls(tuple(lp["x_pos"]), tuple(lp["y_pos"]), tuple(lp["z_pos"]), tuple(lp["switch"]),
+                   float(pf_args["vel"]), float(pf_args["dt"]), T_ambient=float(pf_args.get("T_ambient", 300.0)),
+                   **kw)
+
+    def schedule(self):
+        """(ts, xs, ys, zs, ps, mov_dir) with one entry per step."""
+        import numpy as np
+        if getattr(self, "_sched", None) is not None:
+            return self._sched
+        cx, cy, cz = (np.asarray(v, dtype=np.float64) for v in (self.x_pos, self.y_pos, self.z_pos))
+        ts, xs, ys, zs, ps, dirs = [], [], [], [], [], []
+        t0 = 0.0
+        for i in range(len(cx) - 1):
+            seg = np.array([cx[i + 1] - cx[i], cy[i + 1] - cy[i], cz[i + 1] - cz[i]])
+            dist = np.linalg.norm(seg)
+            duration = dist / self.vel
+            t_seg = np.arange(t0, t0 + duration, self.dt)
+            n = len(t_seg)
+            ts.append(t_seg)
+            xs.append(np.linspace(cx[i], cx[i + 1], n))
+            ys.append(np.linspace(cy[i], cy[i + 1], n))
+            zs.append(np.linspace(cz[i], cz[i + 1], n))
+            ps.append(np.full(n, float(self.switch[i])))
+            dirs.append(np.repeat((seg / dist)[None, :], n, axis=0))
+            t0 = t0 + duration
+        self._sched = (np.concatenate(ts), np.concatenate(xs), np.concatenate(ys), np.concatenate(zs),
+                       np.concatenate(ps), np.concatenate(dirs))
+        return self._sched
+
+    def __len__(self):
+        return len(self.schedule()[0])
+
+    def laser(self, i):
+        """The temperature descriptor of step i (valid at t = ts[i])."""
+        ts, xs, ys, zs, ps, dirs = self.schedule()
+        if self.apply_switch and ps[i] == 0.0:
+            return Quench(T_ref=self.T_ambient, grad_T=0.0, cool_rate=0.0)
+        d = dirs[i]
+        vt = self.vel * ts[i]
+        pos0 = (xs[i] - vt * d[0], ys[i] - vt * d[1], zs[i] - vt * d[2])
+        return Rosenthal(pos0=tuple(float(v) for v in pos0), vel=self.vel, Q=self.Q, alpha=self.alpha,
+                         kappa_T=self.kappa_T, T_ambient=self.T_ambient, dir=tuple(float(v) for v in d),
+                         T_cap=self.T_cap)
+
