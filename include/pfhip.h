@@ -143,6 +143,12 @@ int pf_temperature(void* T_dev, int32_t T_dtype, const pf_params_t* p, double t,
  * One-hot initial condition: eta[c, :] = 0, eta[c, cell_ori_inds[c]] = 1.
  * Replaces PFSolver.ini_cond (allen_cahn.py:145-153).  cell_ori_inds_dev: int32[N].
  */
+/* Temperature for one-way CFD coupling (applications/phase_field/coupled/example.py:31-42):
+ * T_out[(z, y, x)] = r * T_past[(x, y, z)] + (1 - r) * T_future[(x, y, z)], r = (t_cfd - t_pf) / dt_cfd,
+ * i.e. the reference's interpolate_T followed by convert_temperature (CFD arrays are (Nx, Ny, Nz),
+ * x slowest; the result is in cell order, x fastest, ready for PF_T_BUFFER). */
+int pf_cfd_temperature(const void* T_past_dev, const void* T_future_dev, int32_t T_dtype, int32_t nz, int32_t ny,
+                       int32_t nx, double t_pf, double t_cfd, double dt_cfd, void* T_out_dev, void* stream);
 int pf_init_onehot(void* eta_dev, int32_t dtype, const int32_t* cell_ori_inds_dev,
                    int64_t n_cells, int32_t k, void* stream);
 

@@ -447,6 +447,18 @@ __global__ __launch_bounds__(256) void reduce_final_kernel(int nb, long long nch
   }
 }
 
+// CFD coupling (coupled/example.py:31-42): T_pf[(z, y, x)] = r T_past[(x, y, z)] + (1 - r) T_future[(x, y, z)]
+template <class TT>
+__global__ __launch_bounds__(256) void cfd_temperature_kernel(const TT* __restrict__ past, const TT* __restrict__ fut,
+                                                             int nz, int ny, int nx, double r, TT* __restrict__ out) {
+  const long long n = (long long)nz * ny * nx;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int ix = (int)(c % nx), iy = (int)((c / nx) % ny), iz = (int)(c / ((long long)nx * ny));
+  const long long src = ((long long)ix * ny + iy) * nz + iz;
+  out[c] = (TT)(r * (double)past[src] + (1.0 - r) * (double)fut[src]);
+}
+
 static bool grid_ok(long long n, long long threads_per_item) {
   return (n + 255) / 256 * threads_per_item < (1LL << 31);
 }
@@ -556,6 +568,27 @@ int pf_temperature(void* T_dev, int32_t T_dtype, const pf_params_t* p, double t,
     temperature_kernel<float><<<g, 256, 0, (hipStream_t)stream>>>(a, (float*)T_dev, t);
   hipError_t err = hipGetLastError();
   return err == hipSuccess ? PF_OK : hip_fail(err, "pf_temperature launch");
+}
+
+int pf_cfd_temperature(const void* T_past_dev, const void* T_future_dev, int32_t T_dtype, int32_t nz, int32_t ny,
+                       int32_t nx, double t_pf, double t_cfd, double dt_cfd, void* T_out_dev, void* stream) {
+  if (!T_past_dev || !T_future_dev || !T_out_dev) return fail(PF_E_ARG, "NULL buffer");
+  if (T_dtype != PF_F64 && T_dtype != PF_F32) return fail(PF_E_ARG, "T dtype must be PF_F64 or PF_F32");
+  if (nz < 1 || ny < 1 || nx < 1) return fail(PF_E_ARG, "grid dimensions must be >= 1");
+  if (!(dt_cfd != 0.0)) return fail(PF_E_ARG, "dt_cfd must be non-zero");
+  const long long n = (long long)nz * ny * nx;
+  if (!grid_ok(n, 1)) return fail(PF_E_ARG, "grid too large");
+  const double r = (t_cfd - t_pf) / dt_cfd;  // coupled/example.py:40
+  const dim3 g((unsigned)((n + 255) / 256));
+  hipStream_t s = (hipStream_t)stream;
+  if (T_dtype == PF_F64)
+    cfd_temperature_kernel<double><<<g, 256, 0, s>>>((const double*)T_past_dev, (const double*)T_future_dev, nz, ny,
+                                                     nx, r, (double*)T_out_dev);
+  else
+    cfd_temperature_kernel<float><<<g, 256, 0, s>>>((const float*)T_past_dev, (const float*)T_future_dev, nz, ny, nx,
+                                                    r, (float*)T_out_dev);
+  hipError_t err = hipGetLastError();
+  return err == hipSuccess ? PF_OK : hip_fail(err, "pf_cfd_temperature launch");
 }
 
 int pf_init_onehot(void* eta_dev, int32_t dtype, const int32_t* ori_dev, int64_t n_cells, int32_t k, void* stream) {

@@ -62,6 +62,7 @@ SIGNATURES = {
     "pf_tile_count": (ctypes.c_int, [_I32, _I32]),
     "pf_tile_shape": (ctypes.c_int, [_I32, _I32, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "pf_tile_lanes": (ctypes.c_int, [_I32, _I32, _I32]),
+    "pf_cfd_temperature": (ctypes.c_int, [_VP, _VP, _I32, _I32, _I32, _I32, _D, _D, _D, _VP, _VP]),
     "pf_segment": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ParamsT), _D, _VP, _VP, _VP, _VP, ctypes.c_int64,
                                   ctypes.POINTER(_I32), _VP]),
     "pf_segment_moments": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ParamsT), _I32, _VP, _VP]),

@@ -211,6 +211,27 @@ class PFSolver:
                 callback(i + 1, a)
         return a
 
+    def save_checkpoint(self, path, eta, t, step=0):
+        """Full-state checkpoint (no counterpart in the reference): eta in its storage dtype (BF16 as
+        its 16-bit pattern), t, step and the parameters, in an .npz file (no pickled objects)."""
+        import json
+        e = self.state_rhs.check_state(eta).detach()
+        raw = e.view(torch.int16) if e.dtype == torch.bfloat16 else e
+        np.savez(path, eta=raw.cpu().numpy(), dtype=str(e.dtype).replace("torch.", ""), t=float(t), step=int(step),
+                 shape=np.array(e.shape), pf_args=json.dumps({k: v for k, v in self.pf_args.items()
+                                                             if isinstance(v, (int, float, str, bool))}))
+        return path
+
+    def load_checkpoint(self, path):
+        """(eta on this solver's device, t, step) from save_checkpoint."""
+        z = np.load(path, allow_pickle=False)
+        dt = str(z["dtype"])
+        raw = torch.as_tensor(z["eta"])
+        eta = raw.view(torch.bfloat16) if dt == "bfloat16" else raw
+        if tuple(eta.shape) != (self.N, self.K):
+            raise ValueError(f"checkpoint eta has shape {tuple(eta.shape)}, expected {(self.N, self.K)}")
+        return self.state_rhs.check_state(eta.to(self.device)), float(z["t"]), int(z["step"])
+
     def ini_cond(self):
         """allen_cahn.py:145-153: one-hot eta[c, cell_ori_inds[c]] = 1."""
         eta = torch.empty((self.N, self.K), dtype=self.dtype, device=self.device)
@@ -295,14 +316,15 @@ class PFSolver:
         print("\nWrite phase-field sols to file...\n")
         step = step // self.pf_args['write_sol_interval']
         Th = self.temperature(T, t)
-        liquid = Th.reshape(-1) > self.pf_args['T_liquidus']
-        inds = self.argmax(self.state_rhs.check_state(pf_sol)).cpu().numpy()
-        rgb = np.asarray(self.polycrystal.unique_oris_rgb)
-        cell_data = {"T": Th.reshape(-1).astype(np.float32), "ori_inds": inds.astype(np.int32)}
+        # argmax and IPF colours on the device; liquid cells black (allen_cahn.py:186-199)
+        inds_d = self.argmax(self.state_rhs.check_state(pf_sol)).long()
+        liquid_d = torch.as_tensor(Th.reshape(-1) > self.pf_args['T_liquidus'], device=self.device)
+        rgb_d = torch.as_tensor(np.asarray(self.polycrystal.unique_oris_rgb), dtype=torch.float64, device=self.device)
+        cell_data = {"T": Th.reshape(-1).astype(np.float32), "ori_inds": inds_d.int().cpu().numpy()}
         for j, nm in enumerate(("ipf_x", "ipf_y", "ipf_z")):
-            c = np.take(rgb[j], inds, axis=0)
-            c[liquid] = 0.
-            cell_data[nm] = c
+            c = rgb_d[j].index_select(0, inds_d)
+            c[liquid_d] = 0.
+            cell_data[nm] = c.cpu().numpy()
         path = os.path.join(self.pf_args['data_dir'], f"vtk/pf/sols/u{step:03d}.vtu")
         os.makedirs(os.path.dirname(path), exist_ok=True)
         pc = self.polycrystal

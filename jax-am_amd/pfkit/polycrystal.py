@@ -148,3 +148,38 @@ class Field(Polycrystal):
         super().__init__(Nx, Ny, Nz, h_xyz, pf_args["num_grains"], pf_args["num_oris"],
                          seed=pf_args.get("id", 1), quat=ori2)
         pf_args["Nx"], pf_args["Ny"], pf_args["Nz"] = self.Nx, self.Ny, self.Nz
+        if "data_dir" in pf_args:
+            self.write_quat(pf_args["data_dir"])
+
+    def write_quat(self, data_dir):
+        """utils.py:111-114: the orientations as quaternions (w, x, y, z) in numpy/pf/quat.npy."""
+        import os
+        folder = os.path.join(data_dir, "numpy/pf")
+        os.makedirs(folder, exist_ok=True)
+        path = os.path.join(folder, "quat.npy")
+        np.save(path, self.quat)
+        return path
+
+
+def stack_layer(below, above):
+    """Multi-layer build (utils.py:151-179, generate_new_mesh): the polycrystal ``above`` placed on
+    top of ``below`` (same Nx, Ny, h and orientation set).  The result has Nz = below.Nz +
+    above.Nz; its cells are those of ``below`` followed by those of ``above`` (C order, z slowest),
+    with the upper layer's centroids shifted by below's height, as the reference concatenates
+    its cell arrays."""
+    import copy
+    if (below.Nx, below.Ny) != (above.Nx, above.Ny) or tuple(below.mesh_h_xyz) != tuple(above.mesh_h_xyz):
+        raise ValueError("layers must share Nx, Ny and the cell size")
+    if below.num_oris != above.num_oris:
+        raise ValueError("layers must share the orientation set")
+    out = copy.copy(below)
+    out.Nz = below.Nz + above.Nz
+    dz = below.Nz * below.mesh_h_xyz[2]
+    upper = np.array(above.centroids, dtype=np.float64)
+    upper[:, 2] += dz
+    out.centroids = np.concatenate((below.centroids, upper), axis=0)
+    out.cell_ori_inds = np.concatenate((np.asarray(below.cell_ori_inds), np.asarray(above.cell_ori_inds)))
+    if hasattr(below, "cell_grain_inds") and hasattr(above, "cell_grain_inds"):
+        out.cell_grain_inds = np.concatenate((below.cell_grain_inds, above.cell_grain_inds + below.num_grains))
+        out.num_grains = below.num_grains + above.num_grains
+    return out

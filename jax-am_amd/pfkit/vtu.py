@@ -50,3 +50,21 @@ def write_hex_vtu(path, Nx, Ny, Nz, h_xyz, cell_data):
                 + _array("types", types) + "</Cells>\n")
         f.write("<CellData>" + "".join(cd) + "</CellData>\n")
         f.write("</Piece>\n</UnstructuredGrid>\n</VTKFile>\n")
+
+
+def read_cell_data(path):
+    """Cell arrays of a file written by write_hex_vtu (name -> (N,) or (N, ncomp) array), as the
+    reference's process_eta reads them back with meshio (utils.py:181-189)."""
+    import re
+    rev = {v: k for k, v in _VTK_TYPES.items()}
+    txt = open(path).read()
+    body = txt[txt.index("<CellData>"):txt.index("</CellData>")]
+    out = {}
+    for m in re.finditer(r'<DataArray type="(\w+)" Name="([^"]+)" NumberOfComponents="(\d+)" format="binary">'
+                         r'([^<]*)</DataArray>', body):
+        typ, name, ncomp, data = m.group(1), m.group(2), int(m.group(3)), m.group(4)
+        raw = base64.b64decode(data)
+        nbytes = struct.unpack("<Q", raw[:8])[0]
+        a = np.frombuffer(raw[8:8 + nbytes], dtype=rev[typ]).copy()
+        out[name] = a if ncomp == 1 else a.reshape(-1, ncomp)
+    return out

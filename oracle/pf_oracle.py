@@ -127,6 +127,18 @@ def explicit_euler(eta, t_prev, t_crt, T, P, h_xyz, grain_dirs=None):
     return y
 
 
+def rk4(eta, t_prev, t_crt, T, P, h_xyz, grain_dirs=None):
+    """rk4, allen_cahn.py:88-101: classical Runge-Kutta with the temperature T held fixed over the
+    step (the reference passes T through ode_params); no liquid reset."""
+    h = t_crt - t_prev
+    f = lambda y: state_rhs(y, T, P, h_xyz, grain_dirs)  # noqa: E731
+    k1 = h * f(eta)
+    k2 = h * f(eta + k1 / 2.)
+    k3 = h * f(eta + k2 / 2.)
+    k4 = h * f(eta + k3)
+    return eta + 1. / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+
+
 # --------------------------------------------------------------------- temperature
 def centroids(Nz, Ny, Nx, h_xyz):
     """Cell centroids (N, 3) in (x, y, z), cells in C order (z, y, x), x fastest (utils.py:43-57)."""
