@@ -19,6 +19,7 @@ import os
 import sys
 import threading
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -60,12 +61,32 @@ def _load():
     return _cache, _table
 
 
-def lookup(pf_dtype, pf_args, aniso):
-    """(tile, zchunk) for this problem: user cache, then the shipped table, then (-1, 0)."""
-    k = key(pf_dtype, pf_args["num_oris"], aniso, pf_args["Nz"], pf_args["Ny"], pf_args["Nx"])
+# Shape class whose entries are used for throughput (batched) execution: when many instances
+# share the GPU, parallelism is abundant and the shapes that are fastest on a large grid win,
+# whatever the size of each instance.
+THROUGHPUT_CLASS = (64, 256, 256)
+
+
+def batch_key(pf_dtype, K, aniso):
+    return f"batch_d{pf_dtype}_k{int(K)}_a{int(bool(aniso))}"
+
+
+def lookup(pf_dtype, pf_args, aniso, throughput=False):
+    """(tile, zchunk) for this problem: user cache, then the shipped table, then (-1, 0).
+    throughput=True (instances run in a batch): the entry tuned on batches of this (dtype, K,
+    anisotropy), else the large-grid class."""
+    keys = []
+    if throughput:
+        keys.append(batch_key(pf_dtype, pf_args["num_oris"], aniso))
+    dims = THROUGHPUT_CLASS if throughput else (pf_args["Nz"], pf_args["Ny"], pf_args["Nx"])
+    keys.append(key(pf_dtype, pf_args["num_oris"], aniso, *dims))
+    e = None
     with _lock:
         cache, table = _load()
-        e = cache.get(k) or table.get(k)
+        for k in keys:
+            e = cache.get(k) or table.get(k)
+            if e is not None:
+                break
     if e is None:
         return -1, 0
     return int(e["tile"]), int(e["zchunk"])
@@ -150,6 +171,43 @@ def _tune_one(dtype, K, aniso, Nz, Ny, Nx, states=(20, 400), log=None):
     return k, entry
 
 
+def _tune_batch(dtype, K, aniso, n_inst=12, warm=30, steps=60, seed=7, log=None):
+    """Best (tile, zchunk) for batched execution: a batch of n_inst instances of this (dtype, K,
+    anisotropy) with grids drawn log-uniformly between 32^3 and 96x96x64, advanced together by the
+    batch engine (graphs of 20 steps); each candidate is timed over `steps` steps after `warm`."""
+    import time
+    from . import workloads
+    from .batch import BatchEngine
+    rng = np.random.default_rng(seed)
+    wls = []
+    for j in range(n_inst):
+        u = rng.random()
+        Nx = int(round(32 * 3.0 ** u)); Nz = int(round(32 * 2.0 ** u))
+        wls.append(workloads.make(f"tb{j}", Nx, Nx, Nz, K, max(8, Nx * Nx * Nz // 8000), 10 ** 6, dtype,
+                                  0.15 if aniso else 0.0, seed=seed * 100 + j, device_grains=True))
+    pf = _lib.TORCH_TO_PF[_dtype(dtype)]
+    lib = _lib.load()
+    times = {}
+    for cand in [(t, z) for t in range(max(lib.pf_tile_count(pf, K), 1)) for z in (0, 8, 16)]:
+        eng = BatchEngine(max_instances=n_inst + 4)
+        for w in wls:
+            eng.add(w.instance(tile=cand[0], zchunk=cand[1]))
+        eng.run(warm, 20)
+        eng.stream.synchronize()
+        t0 = time.perf_counter()
+        eng.run(steps, 20)
+        eng.stream.synchronize()
+        times[cand] = (time.perf_counter() - t0) / steps * 1e3
+        eng.close()
+    best = min(times, key=times.get)
+    k = batch_key(pf, K, aniso)
+    entry = dict(tile=best[0], zchunk=best[1], ms=round(times[best], 5), default_ms=round(times[(0, 0)], 5),
+                 tuned_on=f"batch of {n_inst} instances, 32^3 .. 96x96x64")
+    if log:
+        log(f"{k}: tile {best[0]} zchunk {best[1]} {times[best]:.4f} ms/step (default {times[(0, 0)]:.4f})")
+    return k, entry
+
+
 def sweep(out_path, Ks=(10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20), log=None):
     """Tune the shipped table: c1/c2/FP64 shapes plus the batched/serving classes."""
     entries = {}
@@ -168,9 +226,15 @@ def sweep(out_path, Ks=(10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20), log=None):
             continue
         k, e = _tune_one(dtype, K, aniso, Nz, Ny, Nx, log=log)
         entries[k] = e
+    for K in Ks:
+        for dtype in ("float32", "bfloat16"):
+            k, e = _tune_batch(dtype, K, True, log=log)
+            entries[k] = e
     doc = {"device": torch.cuda.get_device_name(0), "library_abi": int(_lib.load().pf_version()),
-           "method": "fastest of all (tile, zchunk) candidates by the mean time of 20 timed steps (after 3 "
-                     "warm-up steps) on the states reached after 20 and 400 steps of the workload's one-hot start",
+           "method": "single-instance entries: fastest of all (tile, zchunk) candidates by the mean time of 20 "
+                     "timed steps (after 3 warm-up steps) on the states reached after 20 and 400 steps of the "
+                     "workload's one-hot start; batch_* entries: fastest candidate for a batch of 12 instances "
+                     "advanced together by the batch engine",
            "entries": entries}
     with open(out_path, "w") as f:
         json.dump(doc, f, indent=1, sort_keys=True)

@@ -34,7 +34,7 @@ class Instance:
     t0: float = 0.0
     cell_ori_inds: object = None
     eta0: object = None
-    tile: object = None             # None: the autotuner's choice (pfkit.autotune.lookup)
+    tile: object = None             # None: the autotuner's throughput choice (pfkit.autotune.lookup)
     zchunk: object = None
     tag: object = None
     # filled by the engine
@@ -98,7 +98,7 @@ class BatchEngine:
         bi.dtype = _lib.TORCH_TO_PF[tdt]
         if inst.tile is None or inst.zchunk is None:
             from . import autotune
-            t, z = autotune.lookup(_lib.TORCH_TO_PF[tdt], a, float(a["anisotropy"]) > 0)
+            t, z = autotune.lookup(_lib.TORCH_TO_PF[tdt], a, float(a["anisotropy"]) > 0, throughput=True)
             inst.tile = t if inst.tile is None else inst.tile
             inst.zchunk = z if inst.zchunk is None else inst.zchunk
         bi.tile = int(inst.tile)
