@@ -212,11 +212,22 @@ __device__ __forceinline__ float gauss_ez32(const ColdPath& k, int iz) {
 }
 __device__ __forceinline__ float gauss_T32(const ColdPath& k, float exy, float ez) { return k.T_amb + k.dT * (exy * ez); }
 
+// Global load whose value is forced into its register on the spot.  The compiler's wait
+// analysis does not follow branch conditions: a load left outstanding on the temperature-buffer
+// branch would make the other temperature models wait, where the branches join, for every
+// outstanding memory operation, the march's prefetched plane and row stores included.
+template <class T>
+__device__ __forceinline__ T load_now(const T* p) {
+  T v = *p;
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // Temperature at the centroid of cell (ix, iy, iz) at time t (FP64); cell index c for the buffer mode.
 __device__ __forceinline__ double cell_temperature(const StepDev& a, int ix, int iy, int iz, long long c,
                                                    double t) {
   if (a.tmodel == PF_T_BUFFER) {
-    return a.t_f64 ? ((const double*)a.T)[c] : (double)((const float*)a.T)[c];
+    return a.t_f64 ? load_now((const double*)a.T + c) : (double)load_now((const float*)a.T + c);
   }
   const double cx = ((double)ix + 0.5) * a.hx;
   const double cy = ((double)iy + 0.5) * a.hy;
@@ -247,7 +258,7 @@ __device__ __forceinline__ double cell_temperature(const StepDev& a, int ix, int
 // The same temperature in FP32 (cold-path test and value).
 __device__ __forceinline__ float cell_temperature_f32(const StepDev& a, const ColdPath& k, int ix, int iy, int iz,
                                                       long long c) {
-  if (a.tmodel == PF_T_BUFFER) return a.t_f64 ? (float)((const double*)a.T)[c] : ((const float*)a.T)[c];
+  if (a.tmodel == PF_T_BUFFER) return a.t_f64 ? (float)load_now((const double*)a.T + c) : load_now((const float*)a.T + c);
   const float cz = ((float)iz + 0.5f) * k.hz;
   if (a.tmodel == PF_T_QUENCH) return k.T_ref + k.grad_T * cz - k.cool_t;
   const float X = ((float)ix + 0.5f) * k.hx - k.px;
@@ -765,12 +776,18 @@ struct March {
       }
     }
     lds_barrier();
+    // Plane z+2 (prefetched one plane ago) replaces plane z and plane z+3 is requested before
+    // this plane's row stores are issued: loads and stores share one completion counter, so
+    // a wait for the prefetched data (or for its registers) placed after the stores would
+    // also wait for the stores.
+    if (n + 1 < len) {
+      store_plane(cb);
+      if (n + 2 < len) load_plane(zstep(z, 3));
+    }
 #if PF_STORE_MODE != 0
     store_rows(z);
 #endif
     if (n + 1 < len) {
-      store_plane(cb);  // plane z+2 replaces plane z
-      if (n + 2 < len) load_plane(zstep(z, 3));
       lds_barrier();
       read_center(cb, lo);
     }
@@ -821,12 +838,11 @@ struct March {
 
   __device__ __forceinline__ void run() {
     TC r0[KH], r1[KH], r2[KH];
-    // prologue: planes zb, zb+1 into LDS; plane zb-1 (edge-replicated) straight into registers
+    // prologue: plane zb-1 (edge-replicated) straight into registers, then planes zb, zb+1
+    // into LDS.  The register loads go first so that the waits for the LDS planes' data also
+    // retire them: no load of the prologue is then outstanding when the z loop starts, and the
+    // loop's waits only concern the prefetched plane.
     const int z0 = zb;
-    load_plane(z0);
-    store_plane(buf0);
-    load_plane(zstep(z0, 1));
-    store_plane(buf1);
     {
       const int zbh = zstep(z0, -1);
       const int sxo = min(gx, a.nx - 1), syo = min(gy, a.ny - 1);
@@ -842,6 +858,10 @@ struct March {
 #pragma unroll
       for (int e = (KH / VEC) * VEC; e < KH; ++e) r0[e] = to_tc(p[e]);
     }
+    load_plane(z0);
+    store_plane(buf0);
+    load_plane(zstep(z0, 1));
+    store_plane(buf1);
     lds_barrier();
     read_center(buf0, r1);
     read_center(buf1, r2);
