@@ -38,6 +38,9 @@ int make_stepdev(const pf_params_t* p, int dtype, const void* T_dev, const void*
   if (n * p->num_oris >= (1LL << 40)) return fail(PF_E_ARG, "grid too large");
   if ((long long)p->ny * p->nx * p->num_oris >= (1LL << 31))
     return fail(PF_E_ARG, "one z-plane must hold fewer than 2^31 values");
+  // the march addresses a z-plane with unsigned 32-bit byte offsets
+  if ((long long)p->ny * p->nx * p->num_oris * (long long)dtype_size(dtype) >= (1LL << 32))
+    return fail(PF_E_ARG, "one z-plane must hold fewer than 2^32 bytes");
   for (int d = 0; d < 3; ++d)
     if (!(p->h[d] > 0.0)) return fail(PF_E_ARG, "mesh_h_xyz must be positive");
   if (!(p->ad_hoc != 0.0)) return fail(PF_E_ARG, "ad_hoc must be non-zero");

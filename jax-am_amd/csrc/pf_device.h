@@ -582,12 +582,29 @@ struct March {
   TS* __restrict__ stage;
   const TC* __restrict__ ax;  // LDS axis table (ANISO)
   const ZTab* __restrict__ zt;  // LDS Gaussian z-factors of this march's planes (gsep)
-  int src_off[G::CPT];
+  unsigned src_off[G::CPT];  // byte offsets within an input plane (unsigned 32-bit: base + offset addressing)
   int dst_off[G::CPT];
   SV pre[G::CPT];
-  int tsrc_off[G::CPTT > 0 ? G::CPTT : 1];
+  unsigned tsrc_off[G::CPTT > 0 ? G::CPTT : 1];
   int tdst_off[G::CPTT > 0 ? G::CPTT : 1];
   TLv pret[G::CPTT > 0 ? G::CPTT : 1];
+  // Row write-back (store_rows): each contiguous (y, z) row segment of the tile goes out in
+  // CH-byte pieces, NJO pieces per thread; opos[j] is the byte offset of this thread's
+  // piece j within an output plane (kNoPiece: none), fixed for the whole march.
+  static constexpr int CELLB = K * (int)sizeof(TS);
+  static constexpr int ROWB = TX * CELLB;
+  // rows whose full length is a multiple of 16 bytes go out in 16-byte pieces whatever the
+  // cell alignment (a partial row's last piece in units of U bytes); otherwise the widest
+  // power-of-two piece dividing a cell
+  static constexpr int CA = CELLB & -CELLB;
+  static constexpr bool WIDE = ROWB % 16 == 0 && CA < 16;
+  static constexpr int CH = WIDE ? 16 : (CA >= 16 ? 16 : CA);
+  static constexpr int U = CA < 4 ? CA : 4;
+  static constexpr int CPR = ROWB / CH;
+  static constexpr int NCHO = TY * CPR;
+  static constexpr int NJO = (NCHO + G::NT - 1) / G::NT;
+  static constexpr unsigned kNoPiece = ~0u;
+  unsigned opos[PF_STORE_MODE != 0 ? NJO : 1];
   int gx, gy, cl, ch0, own, oxm, oxp, oym, oyp, zb, len, x0, y0;
   bool active, gsep;
   double exy64;
@@ -632,7 +649,7 @@ struct March {
       const int r = cs / G::PX, cc = cs % G::PX;
       const int sy = clampi(y0 - 1 + r, 0, a.ny - 1);
       const int sx = clampi(x0 - 1 + cc, 0, a.nx - 1);
-      src_off[j] = (sy * a.nx + sx) * K + v * VEC;
+      src_off[j] = (unsigned)((sy * a.nx + sx) * K + v * VEC) * (unsigned)sizeof(TS);
       dst_off[j] = (w < G::NCH) ? cs * KP + v * VEC : -1;
     }
     if constexpr (TAIL > 0) {
@@ -643,10 +660,24 @@ struct March {
         const int r = cs / G::PX, cc = cs % G::PX;
         const int sy = clampi(y0 - 1 + r, 0, a.ny - 1);
         const int sx = clampi(x0 - 1 + cc, 0, a.nx - 1);
-        tsrc_off[j] = (sy * a.nx + sx) * K + G::CPC * VEC;
+        tsrc_off[j] = (unsigned)((sy * a.nx + sx) * K + G::CPC * VEC) * (unsigned)sizeof(TS);
         tdst_off[j] = (w < G::NCT) ? cs * KP + G::CPC * VEC : -1;
       }
     }
+#if PF_STORE_MODE != 0
+    {
+      const int rowb = min(TX, a.nx - x0) * CELLB;
+#pragma unroll
+      for (int j = 0; j < NJO; ++j) {
+        const int w = tid + j * G::NT;
+        const int r = w / CPR;
+        const int off = (w - r * CPR) * CH;
+        opos[j] = (w < NCHO && y0 + r < a.ny && off < rowb)
+                      ? (unsigned)((y0 + r) * a.nx + x0) * (unsigned)CELLB + (unsigned)off
+                      : kNoPiece;
+      }
+    }
+#endif
     own = ((ly + 1) * G::PX + (lx + 1)) * KP + ch0;
     oxm = own - KP;
     oxp = own + KP;
@@ -659,10 +690,10 @@ struct March {
   __device__ __forceinline__ void load_plane(int z) {
     const TS* p = (const TS*)d.in + (long long)z * plane_elems();
 #pragma unroll
-    for (int j = 0; j < G::CPT; ++j) pre[j] = *(const SV*)(p + src_off[j]);
+    for (int j = 0; j < G::CPT; ++j) pre[j] = *(const SV*)((const char*)p + src_off[j]);
     if constexpr (TAIL > 0) {
 #pragma unroll
-      for (int j = 0; j < G::CPTT; ++j) pret[j] = *(const TLv*)(p + tsrc_off[j]);
+      for (int j = 0; j < G::CPTT; ++j) pret[j] = *(const TLv*)((const char*)p + tsrc_off[j]);
     }
   }
   __device__ __forceinline__ void store_plane(TC* b) {
@@ -796,41 +827,27 @@ struct March {
   // Coalesced write-back of plane z of the tile from the LDS staging rows: consecutive
   // lanes store consecutive CH-byte pieces of each contiguous (y, z) row segment.
   __device__ __forceinline__ void store_rows(int z) {
-    constexpr int CELLB = K * (int)sizeof(TS);
-    constexpr int ROWB = TX * CELLB;
-    // rows whose full length is a multiple of 16 bytes go out in 16-byte pieces whatever the
-    // cell alignment (a partial row's last piece in units of U bytes); otherwise the widest
-    // power-of-two piece dividing a cell
-    constexpr int CA = CELLB & -CELLB;
-    constexpr bool WIDE = ROWB % 16 == 0 && CA < 16;
-    constexpr int CH = WIDE ? 16 : (CA >= 16 ? 16 : CA);
-    constexpr int U = CA < 4 ? CA : 4;
     using CT = typename RawT<CH>::type;
     using GT = typename std::conditional<WIDE, raw16a2_t, CT>::type;
     using UT = typename RawT<U>::type;
-    constexpr int CPR = ROWB / CH;
-    constexpr int NCHO = TY * CPR;
     const int rowb = min(TX, a.nx - x0) * CELLB;
-    TS* out = (TS*)d.out;
+    TS* out = (TS*)d.out + (long long)z * plane_elems();
 #pragma unroll
-    for (int j = 0; j < (NCHO + G::NT - 1) / G::NT; ++j) {
-      const int w = threadIdx.x + j * G::NT;
-      if (w < NCHO) {
-        const int r = w / CPR;
-        const int off = (w - r * CPR) * CH;
-        if (y0 + r < a.ny && off < rowb) {
-          const char* src = (const char*)stage + r * ROWB + off;
-          char* dst = (char*)(out + (((long long)z * a.ny + y0 + r) * a.nx + x0) * K) + off;
-          if (!WIDE || off + CH <= rowb) {
-            const GT v = *(const CT*)src;
+    for (int j = 0; j < NJO; ++j) {
+      if (opos[j] != kNoPiece) {
+        const int w = threadIdx.x + j * G::NT;
+        const int off = (w % CPR) * CH;
+        const char* src = (const char*)stage + w * CH;  // = row (w / CPR) of the staging rows, byte off
+        char* dst = (char*)out + opos[j];
+        if (!WIDE || off + CH <= rowb) {
+          const GT v = *(const CT*)src;
 #if PF_STORE_MODE == 2
-            __builtin_nontemporal_store(v, (GT*)dst);
+          __builtin_nontemporal_store(v, (GT*)dst);
 #else
-            *(GT*)dst = v;
+          *(GT*)dst = v;
 #endif
-          } else {
-            for (int q = 0; q < rowb - off; q += U) *(UT*)(dst + q) = *(const UT*)(src + q);
-          }
+        } else {
+          for (int q = 0; q < rowb - off; q += U) *(UT*)(dst + q) = *(const UT*)(src + q);
         }
       }
     }
