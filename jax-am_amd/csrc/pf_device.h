@@ -954,15 +954,27 @@ __device__ __forceinline__ void generic_cell(const StepDev& a, const StepDyn& dy
   }
 }
 
-// Tile order of a launch: x fastest, then y, then z-chunks from the top of the domain down.
-// Workgroups are dispatched in index order (dealt round-robin to the XCDs), so the top layer,
-// where a surface heat source makes cells hot and the per-cell FP64 path slow, starts first
-// and the cheaper layers below fill in behind it; every XCD gets a share of each layer.
+// Tile order of a launch: layers of z-chunks from the top of the domain down.  Workgroups are
+// dispatched in index order and dealt round-robin to the 8 XCDs, so the top layer, where a
+// surface heat source makes cells hot and the per-cell FP64 path slow, starts first and the
+// cheaper layers below fill in behind it.  Within a layer, when the tile rows divide by 8,
+// workgroup t goes to tile row 8m + (t mod 8): every XCD then walks whole rows of tiles, so
+// the x-neighbours of a tile (whose halo columns it re-reads) run on the same XCD and share its
+// L2, and each XCD still gets an equal share of every layer.  Otherwise x fastest, then y.
 __device__ __forceinline__ void tile_coords(int t, int tiles_x, int tiles_y, int tiles_z, int& tx, int& ty,
                                             int& tz) {
-  tx = t % tiles_x;
-  ty = (t / tiles_x) % tiles_y;
-  tz = tiles_z - 1 - t / (tiles_x * tiles_y);
+  const int nl = tiles_x * tiles_y;
+  const int layer = t / nl, r = t - layer * nl;
+  tz = tiles_z - 1 - layer;
+  if (tiles_y % 8 == 0) {
+    const int k = r >> 3;
+    const int m = k / tiles_x;
+    tx = k - m * tiles_x;
+    ty = 8 * m + (r & 7);
+  } else {
+    tx = r % tiles_x;
+    ty = r / tiles_x;
+  }
 }
 
 }  // namespace pf

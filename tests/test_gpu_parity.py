@@ -107,9 +107,11 @@ def test_bf16_step_matches_oracle(case, cuda):
     assert np.all(np.abs(got - ref) <= 2.0 ** -8 * np.abs(ref) + 2e-6)
 
 
-@pytest.mark.parametrize("K", [10, 17])
-def test_tile_shapes_and_generic_kernel_agree_bitwise(K, cuda):
-    Nx, Ny, Nz = 37, 21, 19
+# (37, 21, 19): partial tiles in x and y; (40, 64, 9): tile rows divisible by 8, which selects
+# the XCD-interleaved tile order of the single-step launch
+@pytest.mark.parametrize("K,dims", [(10, (37, 21, 19)), (17, (37, 21, 19)), (20, (40, 64, 9))])
+def test_tile_shapes_and_generic_kernel_agree_bitwise(K, dims, cuda):
+    Nx, Ny, Nz = dims
     a = args(Nx, Ny, Nz, K, anisotropy=0.15, force_liquid=True)
     pc = poly(Nx, Ny, Nz, K)
     L = gaussian(Nx, Ny, Nz)
