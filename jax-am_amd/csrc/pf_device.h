@@ -119,7 +119,7 @@ struct ColdPath {
   float px, py, pz;           // source position at t
   float hx, hy, hz;
   float hs, T_lim;            // |hstep * stiff|; cold-path cells have T < T_lim and L(T) * hs < 1e-6
-  float L0, Qg_R;             // L0, Qg / gas_const (Arrhenius)
+  float L0, nql2;             // Arrhenius: L0, -Qg / gas_const * log2(e)
   float hm, hk, eps1, eps2;   // hstep*m_g/ad_hoc, hstep*kappa_g*ad_hoc, 1+eps, 2*eps
   float gamma4;
   float T_amb, dT, gxy, gz, q, pe, dxr, dyr, dzr, T_cap, T_ref, grad_T, cool_t;
@@ -134,8 +134,9 @@ struct StepDyn {
 };
 
 // Cold path: a cell whose FP32 mobility satisfies L(T) * |hstep * stiff| < 1e-6 changes by
-// less than 1e-6 * |eta| per step, so the ~1e-6 relative error of an FP32 mobility moves
-// it by under 1e-12.  Cold cells must also lie at least 10 K below T_liquidus, the
+// less than 1e-6 * |eta| per step, so the relative error of its FP32 mobility (hardware
+// reciprocal and exp2: under 2e-5 while the exponent is above -126, below which the mobility
+// is flushed to zero) moves it by under 2e-11 * |eta|.  Cold cells must also lie at least 10 K below T_liquidus, the
 // liquid-reset threshold and T_clip, so zeta = 1 and no reset are the exact FP64 decisions.
 // No transcendental is evaluated here, so host and device build bitwise the same constants.
 __host__ __device__ inline ColdPath make_cold(const StepDev& a, double t, double hstep) {
@@ -150,7 +151,7 @@ __host__ __device__ inline ColdPath make_cold(const StepDev& a, double t, double
   k.hs = (float)fabs(hstep * a.stiff);
   k.T_lim = (float)(fmin(a.T_l, fmin(a.liquid_T, a.T_clip)) - 10.0);
   k.L0 = (float)a.L0;
-  k.Qg_R = (float)(a.Qg / a.gas_const);
+  k.nql2 = (float)(-(a.Qg / a.gas_const) * 1.4426950408889634);
   k.hm = (float)(hstep * a.m_over_ad);
   k.hk = (float)(hstep * a.kappa_ad);
   k.eps1 = (float)(1.0 + a.eps);
@@ -325,7 +326,7 @@ __device__ __forceinline__ CellCoef<TC> cell_coef_with(const StepDev& a, const C
     return cell_coef64<TC>(a, t64(), S, hstep);
   } else {
     CellCoef<float> r;
-    const float Lg = k.L0 * expf(-k.Qg_R / T32);
+    const float Lg = k.L0 * __builtin_amdgcn_exp2f(k.nql2 * __builtin_amdgcn_rcpf(T32));
     const bool hot = !(T32 < k.T_lim && fabsf(Lg) * k.hs < 1e-6f);  // NaN -> FP64 path
     r.c1 = k.hm * Lg;
     r.c2 = k.hk * Lg;
@@ -696,16 +697,21 @@ struct March {
       for (int j = 0; j < G::CPTT; ++j) pret[j] = *(const TLv*)((const char*)p + tsrc_off[j]);
     }
   }
+  // LDS chunk w sits at w * VEC when cells are unpadded (KP = K, no tail chunk): the offsets
+  // are then immediates off one per-thread base instead of a register per chunk.
+  static constexpr bool DLIN = KP == G::CPC * VEC && TAIL == 0;
   __device__ __forceinline__ void store_plane(TC* b) {
 #pragma unroll
     for (int j = 0; j < G::CPT; ++j) {
-      if (dst_off[j] >= 0) {
+      const bool ok = DLIN ? (j + 1 < G::CPT || (int)threadIdx.x + j * G::NT < G::NCH) : dst_off[j] >= 0;
+      if (ok) {
+        TC* dst = DLIN ? b + (int)threadIdx.x * VEC + j * G::NT * VEC : b + dst_off[j];
 #pragma unroll
         for (int q = 0; q < VEC / WV; ++q) {
           WVv c;
 #pragma unroll
           for (int e = 0; e < WV; ++e) c.v[e] = to_tc(pre[j].v[q * WV + e]);
-          *(WVv*)(b + dst_off[j] + q * WV) = c;
+          *(WVv*)(dst + q * WV) = c;
         }
       }
     }
