@@ -38,9 +38,6 @@ int make_stepdev(const pf_params_t* p, int dtype, const void* T_dev, const void*
   if (n * p->num_oris >= (1LL << 40)) return fail(PF_E_ARG, "grid too large");
   if ((long long)p->ny * p->nx * p->num_oris >= (1LL << 31))
     return fail(PF_E_ARG, "one z-plane must hold fewer than 2^31 values");
-  // the march addresses a z-plane with unsigned 32-bit byte offsets
-  if ((long long)p->ny * p->nx * p->num_oris * (long long)dtype_size(dtype) >= (1LL << 32))
-    return fail(PF_E_ARG, "one z-plane must hold fewer than 2^32 bytes");
   for (int d = 0; d < 3; ++d)
     if (!(p->h[d] > 0.0)) return fail(PF_E_ARG, "mesh_h_xyz must be positive");
   if (!(p->ad_hoc != 0.0)) return fail(PF_E_ARG, "ad_hoc must be non-zero");
@@ -171,6 +168,13 @@ int default_zchunk(int nz, int ny, int nx, int tx, int ty) {
   if (zc < 4) zc = 4;
   if (zc > nz) zc = nz;
   return zc;
+}
+
+// The tiled kernels (the march) address a z-plane with unsigned 32-bit byte offsets.
+int march_fits(const pf_params_t* p, int dtype) {
+  if ((long long)p->ny * p->nx * p->num_oris * (long long)dtype_size(dtype) >= (1LL << 32))
+    return fail(PF_E_ARG, "the tiled kernels need a z-plane of fewer than 2^32 bytes (tile -2 runs the generic kernel)");
+  return PF_OK;
 }
 
 void set_tiles(StepDev& a, int tx, int ty, int zc) {
@@ -527,6 +531,7 @@ int pf_step(const void* eta_in_dev, void* eta_out_dev, int32_t dtype, const void
   hipError_t err;
   if (e) {
     if (zchunk < 0) return fail(PF_E_ARG, "zchunk must be >= 0");
+    if (march_fits(p, dtype)) return PF_E_ARG;
     set_tiles(a, e->tx, e->ty, zchunk);
     const long long nt = (long long)a.tiles_x * a.tiles_y * a.tiles_z;
     if (nt >= (1LL << 31)) return fail(PF_E_ARG, "too many tiles");

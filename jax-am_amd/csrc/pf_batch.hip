@@ -250,6 +250,7 @@ int pf_batch_add(pf_batch_t* b, const pf_batch_inst_t* inst) {
   int items, tile = -1, grid;
   BatchLaunch fn;
   if (e) {
+    if (march_fits(&inst->p, inst->dtype)) return PF_E_ARG;
     set_tiles(a, e->tx, e->ty, inst->zchunk);
     items = a.tiles_x * a.tiles_y * a.tiles_z;
     tile = e->tile;

@@ -28,6 +28,7 @@ int entry_count(int dtype, int k, bool aniso);
 // default z-chunk for a grid and tile shape
 int default_zchunk(int nz, int ny, int nx, int tx, int ty);
 // fills the tile grid (tiles_x/y/z, zc) for a tile shape
+int march_fits(const pf_params_t* p, int dtype);
 void set_tiles(StepDev& a, int tx, int ty, int zc);
 
 }  // namespace pf

@@ -76,12 +76,8 @@ def test_argument_validation_without_gpu():
     assert lib.pf_batch_create(0, 0, None) is None
     rc = lib.pf_step(16, 40, _lib.PF_F32, None, p, None, 0.0, 1.0, -1, 0, None)
     assert rc == -1 and b"aligned" in lib.pf_last_error()
-    # the march addresses a z-plane with 32-bit byte offsets: 8192 x 8192 x 20 FP32 values
-    # (5.4 GB) per plane are refused; the same plane in BF16 (2.7 GB) passes that check and is
-    # stopped by the invalid tile index (-5), which pf_step checks after the parameters and
-    # before any launch
+    # the tiled kernels address a z-plane with 32-bit byte offsets: a tiled step on 8192 x 8192
+    # x 20 FP32 values (5.4 GB) per plane is refused before any launch
     big = make_params(args(8192, 8192, 1, 20), (1.0, 1.0, 1.0))
-    rc = lib.pf_step(16, 32, _lib.PF_F32, 3, big, None, 0.0, 1.0, -5, 0, None)
+    rc = lib.pf_step(16, 32, _lib.PF_F32, 3, big, None, 0.0, 1.0, 0, 0, None)
     assert rc == -1 and b"2^32 bytes" in lib.pf_last_error()
-    rc = lib.pf_step(16, 32, _lib.PF_BF16, 3, big, None, 0.0, 1.0, -5, 0, None)
-    assert rc == -1 and b"tile must be" in lib.pf_last_error()
