@@ -533,9 +533,17 @@ template <class TS, int K, int TX, int TY, int H> struct Tile {
   static constexpr int NCT = TAIL ? PY * PX : 0;  // tail chunks per plane (one per cell)
   static constexpr int CPTT = (NCT + NT - 1) / NT;
   static constexpr int PLANE = PY * PX * KP;     // TC elements per LDS plane
-  static constexpr int STAGE = PF_STORE_MODE != 0 ? NC * K : 8;  // TS elements of output staging
-  static constexpr int AXN = 12 * K;                              // TC elements of the LDS axis table
-  static constexpr size_t LDS_BYTES = 2 * PLANE * sizeof(TC) + STAGE * sizeof(TS) + AXN * sizeof(TC) + 768;
+  // BF16 lane-pair tiles keep a ring of three LDS planes and store their results straight
+  // from registers: one barrier per plane (c2: -3 %).  Every other tile keeps two planes and
+  // stages its results in LDS for coalesced row stores, two barriers per plane: FP32 pieces
+  // stored per lane (8 bytes at an 80-byte cell stride) cost more than the barrier saves
+  // (c1: 0.142 -> 0.182 ms), and for one-lane tiles a third plane would not fit two
+  // workgroups per CU.
+  static constexpr bool RING3 = H == 2 && sizeof(TS) == 2;
+  static constexpr int NBUF = RING3 ? 3 : 2;
+  static constexpr int STAGE = PF_STORE_MODE != 0 && !RING3 ? NC * K : 8;  // TS elements of output staging
+  static constexpr int AXN = 12 * K;                                        // TC elements of the LDS axis table
+  static constexpr size_t LDS_BYTES = NBUF * PLANE * sizeof(TC) + STAGE * sizeof(TS) + AXN * sizeof(TC) + 768;
 };
 
 // The sum of squares S = sum_i eta_i^2 of a cell is formed as two sequential FMA chains, over
@@ -580,6 +588,7 @@ struct March {
   const StepDyn& d;
   TC* __restrict__ buf0;
   TC* __restrict__ buf1;
+  TC* __restrict__ buf2;  // third plane of the ring (RING3)
   TS* __restrict__ stage;
   const TC* __restrict__ ax;  // LDS axis table (ANISO)
   const ZTab* __restrict__ zt;  // LDS Gaussian z-factors of this march's planes (gsep)
@@ -627,6 +636,7 @@ struct March {
     len = min(zb + a.zc, a.nz) - zb;
     buf0 = smem;
     buf1 = smem + G::PLANE;
+    buf2 = smem + (G::RING3 ? 2 : 1) * G::PLANE;
     stage = stage_;
     ax = ax_;
     if constexpr (ANISO) stage_axes<TC, K>((const TC*)a.axes, ax_, tid, G::NT);
@@ -757,12 +767,15 @@ struct March {
   __device__ __forceinline__ int zstep(int z, int d) const { return clampi(z + d, 0, a.nz - 1); }
 
   // Compute plane z = zb + n from (lo, c, hi) = planes z-1, z, z+1, then (if another plane
-  // follows) rotate the LDS ring and read plane z+2 into `lo`.
-  __device__ __forceinline__ void body(int n, TC (&lo)[KH], TC (&c)[KH], TC (&hi)[KH]) {
+  // follows) put plane z+2 into LDS and read its centre into `lo`.  cb holds plane z.  Two
+  // planes: plane z+2 replaces plane z in cb after a barrier.  Ring of three (RING3): plane
+  // z+2 goes to tb, the plane read while computing z-1 (done before the previous barrier).
+  __device__ __forceinline__ void body(int n, TC (&lo)[KH], TC (&c)[KH], TC (&hi)[KH], TC* cb, TC* tb) {
     const int z = zb + n;
-    TC* cb = (n & 1) ? buf1 : buf0;
+    TC res[KH];
+    long long cell = 0;
     if (active) {
-      const long long cell = ((long long)z * a.ny + gy) * a.nx + gx;
+      cell = ((long long)z * a.ny + gy) * a.nx + gx;
       const TC S = cell_sum_sq(c);
       float T32 = 0.0f;
       if constexpr (sizeof(TC) == 4)
@@ -774,7 +787,6 @@ struct March {
       const TC B = lap_B<TC>(a);
       TC ihx2, ihy2, ihz2;
       inv_h2<TC>(a, ihx2, ihy2, ihz2);
-      TC res[KH];
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         const LVv vxm = *(const LVv*)(cb + oxm + g * LV);
@@ -799,34 +811,53 @@ struct March {
 #pragma unroll
         for (int i = 0; i < KH; ++i) res[i] = cf.liquid ? (TC)0 : res[i];
       }
+      if constexpr (!G::RING3) {
 #if PF_STORE_MODE == 0
-      TS* o = (TS*)d.out + cell * K + ch0;
+        TS* o = (TS*)d.out + cell * K + ch0;
 #else
-      TS* o = stage + cl * K + ch0;
+        TS* o = stage + cl * K + ch0;
 #endif
-#pragma unroll
-      for (int j = 0; j < KH / G::SVS; ++j) {
-        SVv s;
-#pragma unroll
-        for (int e = 0; e < G::SVS; ++e) s.v[e] = from_tc<TS>(res[j * G::SVS + e]);
-        *(SVv*)(o + j * G::SVS) = s;
+        store_cell(o, res);
       }
     }
-    lds_barrier();
-    // Plane z+2 (prefetched one plane ago) replaces plane z and plane z+3 is requested before
-    // this plane's row stores are issued: loads and stores share one completion counter, so
-    // a wait for the prefetched data (or for its registers) placed after the stores would
-    // also wait for the stores.
-    if (n + 1 < len) {
-      store_plane(cb);
-      if (n + 2 < len) load_plane(zstep(z, 3));
-    }
-#if PF_STORE_MODE != 0
-    store_rows(z);
-#endif
-    if (n + 1 < len) {
+    if constexpr (G::RING3) {
+      // Plane z+2 into tb and plane z+3 requested before this plane's stores are issued (loads
+      // and stores share one completion counter), then the results go out from registers.
+      if (n + 1 < len) {
+        store_plane(tb);
+        if (n + 2 < len) load_plane(zstep(z, 3));
+      }
+      if (active) store_cell((TS*)d.out + cell * K + ch0, res);
       lds_barrier();
-      read_center(cb, lo);
+      if (n + 1 < len) read_center(tb, lo);
+    } else {
+      lds_barrier();
+      // Plane z+2 (prefetched one plane ago) replaces plane z and plane z+3 is requested before
+      // this plane's row stores are issued: loads and stores share one completion counter, so
+      // a wait for the prefetched data (or for its registers) placed after the stores would
+      // also wait for the stores.
+      if (n + 1 < len) {
+        store_plane(cb);
+        if (n + 2 < len) load_plane(zstep(z, 3));
+      }
+#if PF_STORE_MODE != 0
+      store_rows(z);
+#endif
+      if (n + 1 < len) {
+        lds_barrier();
+        read_center(cb, lo);
+      }
+    }
+  }
+
+  // This thread's KH results of one cell to o (global memory or the LDS staging rows).
+  __device__ __forceinline__ void store_cell(TS* o, const TC (&res)[KH]) {
+#pragma unroll
+    for (int j = 0; j < KH / G::SVS; ++j) {
+      SVv s;
+#pragma unroll
+      for (int e = 0; e < G::SVS; ++e) s.v[e] = from_tc<TS>(res[j * G::SVS + e]);
+      *(SVv*)(o + j * G::SVS) = s;
     }
   }
 
@@ -890,9 +921,17 @@ struct March {
     read_center(buf1, r2);
     if (len > 1) load_plane(zstep(z0, 2));
     for (int n = 0; n < len; n += 3) {
-      body(n, r0, r1, r2);
-      if (n + 1 < len) body(n + 1, r1, r2, r0);
-      if (n + 2 < len) body(n + 2, r2, r0, r1);
+      if constexpr (G::RING3) {  // plane zb + m lives in buffer m mod 3
+        body(n, r0, r1, r2, buf0, buf2);
+        if (n + 1 < len) body(n + 1, r1, r2, r0, buf1, buf0);
+        if (n + 2 < len) body(n + 2, r2, r0, r1, buf2, buf1);
+      } else {  // plane zb + m lives in buffer m mod 2
+        TC* const e = (n & 1) ? buf1 : buf0;
+        TC* const o = (n & 1) ? buf0 : buf1;
+        body(n, r0, r1, r2, e, e);
+        if (n + 1 < len) body(n + 1, r1, r2, r0, o, o);
+        if (n + 2 < len) body(n + 2, r2, r0, r1, e, e);
+      }
     }
   }
 };
