@@ -135,6 +135,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="step throughput + roofline + CPU baseline only")
     ap.add_argument("--only-step", action="store_true", help="only the timed FP32 steps (for profilers)")
+    ap.add_argument("--only", choices=("bf16", "f64", "batched"), default=None,
+                    help="only the timed c1 BF16 or FP64-mode steps, or only the c3 batched run (for profilers)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tile", type=int, default=None)
     ap.add_argument("--zchunk", type=int, default=None)
@@ -161,8 +163,25 @@ def main():
     import torch
     from pfkit import PFSolver, workloads
     torch.cuda.set_device(0)
+    if args.only == "batched":
+        from pfkit import batchbench
+        log("c3: 64 instances batched and one at a time...")
+        r = batchbench.run_batched(log)
+        print(json.dumps({"only": "batched", **{k: r[k] for k in ("batched_updates_per_s", "one_at_a_time_updates_per_s",
+                                                                  "speedup_vs_one_at_a_time", "bitwise_equal_to_solo")}}))
+        return
     log("building workload c1 (256x256x64x20)...")
     wl = workloads.c1()
+    if args.only in ("bf16", "f64"):
+        dt = {"bf16": "bfloat16", "f64": "float64"}[args.only]
+        so = PFSolver(wl.pf_args, wl.polycrystal, dtype=dt, clean=False)
+        ts = wl.ts(args.warmup + args.steps)
+        log(f"timing {args.steps} {dt} steps after {args.warmup} warm-up steps (tile {so.tile}, zchunk {so.zchunk})")
+        w, e, _ = time_steps(so, so.ini_cond(), wl.laser, ts, args.warmup, args.steps)
+        print(json.dumps({"only": args.only, "ms_per_step": w / args.steps * 1e3, "kernel_ms": e / args.steps,
+                          "roofline_frac": BYTES_PER_UPDATE[dt] * wl.N * wl.K / (e / 1e3 / args.steps) / 1e9
+                          / HBM_PEAK_GBS, "tile": so.tile, "zchunk": so.zchunk}))
+        return
     s = PFSolver(wl.pf_args, wl.polycrystal, dtype="float32", clean=False, tile=args.tile, zchunk=args.zchunk)
     eta0 = s.ini_cond()
     ts = wl.ts(args.warmup + args.steps)
