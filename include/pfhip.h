@@ -212,6 +212,12 @@ typedef struct pf_batch_inst {
   int32_t steps;              /* number of Euler steps to run */
   double t0, dt;              /* start time and fixed step */
   pf_params_t p;
+  /* Optional per-step source table (toolpaths, multi_scan/example.py:52-105), NULL for none:
+   * device array [steps][8] of FP64 rows (t_prev, t_crt, pos0 x, y, z, dir x, y, z).  Step i
+   * then runs from t_prev to t_crt of row i with p.laser.pos0/dir replaced by the row's, exactly
+   * as pf_step called with that row's times and source; t0/dt are then only the reported start.
+   * Needs a Rosenthal or Gaussian source.  The table is the caller's and must outlive the slot. */
+  const double* path_dev;
 } pf_batch_inst_t;
 
 pf_batch_t* pf_batch_create(int32_t max_instances, int32_t max_tiles, void* stream);

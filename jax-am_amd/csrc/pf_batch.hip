@@ -24,17 +24,33 @@
 
 namespace pf {
 
-__global__ __launch_bounds__(64) void advance_kernel(const StepDev* __restrict__ descs, BatchSlot* __restrict__ slots,
+// After each step: advance every active slot's counters and build its next step's arguments;
+// a toolpath slot also takes its next row's times and source position into its description.
+__global__ __launch_bounds__(64) void advance_kernel(StepDev* __restrict__ descs, BatchSlot* __restrict__ slots,
                                                     StepDyn* __restrict__ dyns, int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   BatchSlot b = slots[s];
   if (b.active && b.steps_left > 0) {
+    const int done = b.n;
     b.steps_left -= 1;
     b.parity ^= 1;
     b.n += 1;
-    slots[s] = b;
-    dyns[s] = slot_dyn(descs[s], b);
+    if (b.path) {
+      b.tnow = b.path[8 * done + 1];
+      slots[s] = b;
+      if (b.steps_left > 0) {
+        StepDev a = descs[s];
+        const double* r = b.path + 8 * b.n;
+        apply_path_row(a, r);
+        descs[s] = a;
+        dyns[s] = make_dyn(a, b.eta[b.parity], b.eta[b.parity ^ 1], r[0], r[1] - r[0]);
+      }
+    } else {
+      b.tnow = b.t0 + (double)b.n * b.dt;
+      slots[s] = b;
+      dyns[s] = slot_dyn(descs[s], b);
+    }
   }
 }
 
@@ -235,6 +251,8 @@ int pf_batch_add(pf_batch_t* b, const pf_batch_inst_t* inst) {
     return fail(PF_E_ARG, "eta buffers must be 16-byte aligned");
   if (inst->steps < 0) return fail(PF_E_ARG, "steps must be >= 0");
   if (inst->zchunk < 0) return fail(PF_E_ARG, "zchunk must be >= 0");
+  if (inst->path_dev && inst->p.laser.model != PF_T_ROSENTHAL && inst->p.laser.model != PF_T_GAUSSIAN)
+    return fail(PF_E_ARG, "a source table needs a Rosenthal or Gaussian source");
   int slot = -1;
   for (int s = 0; s < b->max_inst; ++s)
     if (!b->slots[s].used) {
@@ -289,8 +307,19 @@ int pf_batch_add(pf_batch_t* b, const pf_batch_inst_t* inst) {
   st.dt = inst->dt;
   st.eta[0] = inst->eta_dev[0];
   st.eta[1] = inst->eta_dev[1];
-  const StepDyn dy = slot_dyn(a, st);
-  hipError_t he = hipMemcpyAsync(b->d_desc + slot, &a, sizeof(StepDev), hipMemcpyHostToDevice, b->stream);
+  st.tnow = inst->t0;
+  st.path = inst->path_dev;
+  StepDyn dy = slot_dyn(a, st);
+  hipError_t he = hipSuccess;
+  if (inst->path_dev && inst->steps > 0) {  // the first row's times and source
+    double r0[8];
+    he = hipMemcpyAsync(r0, inst->path_dev, sizeof(r0), hipMemcpyDeviceToHost, b->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(b->stream);
+    if (he != hipSuccess) return hip_fail(he, "pf_batch_add path");
+    apply_path_row(a, r0);
+    dy = make_dyn(a, st.eta[0], st.eta[1], r0[0], r0[1] - r0[0]);
+  }
+  he = hipMemcpyAsync(b->d_desc + slot, &a, sizeof(StepDev), hipMemcpyHostToDevice, b->stream);
   if (he == hipSuccess) he = hipMemcpyAsync(b->d_dyn + slot, &dy, sizeof(StepDyn), hipMemcpyHostToDevice, b->stream);
   if (he == hipSuccess)
     he = hipMemcpyAsync(b->d_slots + slot, &st, sizeof(BatchSlot), hipMemcpyHostToDevice, b->stream);
@@ -353,7 +382,7 @@ int pf_batch_status(pf_batch_t* b, int32_t* steps_left, double* t, int32_t* pari
   if (e != hipSuccess) return hip_fail(e, "pf_batch_status");
   for (int s = 0; s < n; ++s) {
     if (steps_left) steps_left[s] = b->slots[s].used ? h[s].steps_left : -1;
-    if (t) t[s] = h[s].t0 + (double)h[s].n * h[s].dt;
+    if (t) t[s] = h[s].tnow;
     if (parity) parity[s] = h[s].parity;
   }
   return PF_OK;

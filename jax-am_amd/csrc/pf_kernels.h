@@ -17,7 +17,20 @@ struct BatchSlot {
   int active;
   double t0, dt;
   void* eta[2];
+  double tnow;         // time of the current state
+  const double* path;  // per-step rows (t_prev, t_crt, pos0 xyz, dir xyz) of a toolpath, or null
 };
+
+// A toolpath row's source position and direction into an instance description (the fields
+// make_stepdev copies from pf_laser_t pos0/dir).
+__host__ __device__ inline void apply_path_row(StepDev& a, const double* r) {
+  a.p0x = r[2];
+  a.p0y = r[3];
+  a.p0z = r[4];
+  a.dirx = r[5];
+  a.diry = r[6];
+  a.dirz = r[7];
+}
 
 struct WorkItem {
   int slot;

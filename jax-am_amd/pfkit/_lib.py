@@ -47,7 +47,8 @@ class ParamsT(ctypes.Structure):
 class BatchInstT(ctypes.Structure):
     _fields_ = [("eta_dev", ctypes.c_void_p * 2), ("T_dev", ctypes.c_void_p), ("axes_dev", ctypes.c_void_p),
                 ("dtype", ctypes.c_int32), ("tile", ctypes.c_int32), ("zchunk", ctypes.c_int32),
-                ("steps", ctypes.c_int32), ("t0", ctypes.c_double), ("dt", ctypes.c_double), ("p", ParamsT)]
+                ("steps", ctypes.c_int32), ("t0", ctypes.c_double), ("dt", ctypes.c_double), ("p", ParamsT),
+                ("path_dev", ctypes.c_void_p)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/pfhip.h

@@ -37,10 +37,12 @@ class Instance:
     tile: object = None             # None: the autotuner's throughput choice (pfkit.autotune.lookup)
     zchunk: object = None
     tag: object = None
+    toolpath: object = None         # pfkit.laser.Toolpath: step i runs ts[i] -> ts[i+1] with its laser(i)
     # filled by the engine
     slot: int = -1
     bufs: list = field(default_factory=list)
     axes: object = None
+    path: object = None             # device table of the toolpath's per-step rows
 
     @property
     def N(self):
@@ -106,11 +108,31 @@ class BatchEngine:
         bi.steps = int(inst.steps)
         bi.t0 = float(inst.t0)
         bi.dt = float(inst.dt)
-        bi.p = make_params(a, inst.h_xyz, inst.laser)
+        laser, path = inst.laser, None
+        if inst.toolpath is not None:
+            # rows (t_prev, t_crt, pos0, dir) of the steps, as PFSolver.run_toolpath passes them to pf_step
+            tp = inst.toolpath
+            if getattr(tp, "apply_switch", False):
+                raise ValueError("batched toolpaths run with the source always on (apply_switch=False)")
+            ts = tp.schedule()[0]
+            if inst.steps > len(ts) - 1:
+                raise ValueError(f"the toolpath has {len(ts) - 1} steps, {inst.steps} asked for")
+            rows = np.empty((max(int(inst.steps), 1), 8))
+            for i in range(int(inst.steps)):
+                L = tp.laser(i)
+                rows[i] = (ts[i], ts[i + 1], *L.pos0, *L.dir)
+            path = torch.as_tensor(rows, dtype=torch.float64, device=self.device)
+            laser = tp.laser(0)
+            bi.t0 = float(ts[0])
+            bi.path_dev = path.data_ptr()
+        else:
+            bi.path_dev = None
+        bi.p = make_params(a, inst.h_xyz, laser)
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))  # the table is copied on the caller's stream
         slot = self.lib.pf_batch_add(self.h, ctypes.byref(bi))
         if slot < 0:
             _lib.check(slot, "pf_batch_add")
-        inst.slot, inst.bufs, inst.axes = slot, [b0, b1], axes
+        inst.slot, inst.bufs, inst.axes, inst.path = slot, [b0, b1], axes, path
         self.insts[slot] = inst
         return slot
 

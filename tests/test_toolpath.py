@@ -76,3 +76,29 @@ def test_run_toolpath_matches_oracle_and_stepper(cuda):
     for i in range(n):
         state, sol = s.stepper(state, ts[i + 1], [tp.laser(i)])
     assert torch.equal(sol, torch.as_tensor(got, device=sol.device))
+
+
+@pytest.mark.gpu
+def test_batched_toolpath_equals_run_toolpath(cuda):
+    """A toolpath instance in the batch engine (per-step source table) next to an ordinary
+    instance: bitwise the result of PFSolver.run_toolpath, for FP64 and FP32."""
+    import torch
+    from pfkit import PFSolver, workloads
+    from pfkit.batch import BatchEngine, Instance
+    a, pc, tp, eta0 = _problem()
+    n = len(tp) - 1
+    ts = tp.schedule()[0]
+    other = workloads.make("other", 20, 16, 12, 10, 12, 9, "float32", 0.15, seed=5, vel=2.0)
+    for dtype in ("float64", "float32"):
+        s = PFSolver(a, pc, dtype=dtype, device=cuda, clean=False)
+        solo = s.run_toolpath(eta0, tp)
+        eng = BatchEngine(max_instances=4)
+        slot = eng.add(Instance(pf_args=a, h_xyz=pc.mesh_h_xyz, grain_dirs=pc.unique_grain_directions, laser=None,
+                                dtype=dtype, steps=n, dt=tp.dt, eta0=eta0, toolpath=tp))
+        slot2 = eng.add(other.instance())
+        eng.run_to_completion(chunk=4)
+        steps_left, t, par = eng.status()
+        assert steps_left[slot] == 0 and steps_left[slot2] == 0
+        assert t[slot] == ts[n]
+        assert torch.equal(eng.state(slot, par[slot]), solo), dtype
+        eng.close()
