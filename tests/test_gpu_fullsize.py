@@ -86,3 +86,25 @@ def test_c1_tile_shapes_and_zchunks_agree_bitwise(c1_state):
         for tile in range(lib.pf_tile_count(_lib.TORCH_TO_PF[dtype], wl.K)):
             for zc in (0, 8, 12, 64):
                 assert torch.equal(step(tile, zc), base), (dtype, tile, zc)
+
+
+def test_c0_full_run_f64_matches_c_oracle(cuda):
+    """The CPU-runnable configuration (64 x 64 x 32 grid, K = 10, 50 grains, isotropic, moving
+    Gaussian source, 200 steps from the one-hot start): the whole FP64 trajectory on the GPU
+    against the C/OpenMP FP64 oracle (relative L-inf <= 1e-10 after 200 steps), and the FP32
+    run against it (max |d eta| <= 1e-5)."""
+    if not C.available():
+        pytest.skip("oracle/_build/libpforacle.so not built (make -C oracle)")
+    wl = workloads.c0()
+    ts = wl.ts()
+    n = len(ts) - 1
+    s64 = PFSolver(wl.pf_args, wl.polycrystal, dtype=torch.float64, clean=False)
+    eta0 = s64.ini_cond()
+    got = s64.run(eta0, ts, wl.laser).cpu().numpy()
+    ref = _oracle_steps(wl, ts, eta0.cpu().numpy(), 0, n)
+    assert n == 200
+    assert np.count_nonzero((ref > 1e-3) & (ref < 1 - 1e-3)) > 1000   # interfaces have moved
+    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
+    s32 = PFSolver(wl.pf_args, wl.polycrystal, dtype=torch.float32, clean=False)
+    got32 = s32.run(s32.ini_cond(), ts, wl.laser).to(torch.float64).cpu().numpy()
+    assert np.max(np.abs(got32 - ref)) <= 1e-5
