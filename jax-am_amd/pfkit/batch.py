@@ -117,7 +117,7 @@ class BatchEngine:
             ts = tp.schedule()[0]
             if inst.steps > len(ts) - 1:
                 raise ValueError(f"the toolpath has {len(ts) - 1} steps, {inst.steps} asked for")
-            rows = np.empty((max(int(inst.steps), 1), 8))
+            rows = np.zeros((max(int(inst.steps), 1), 8))
             for i in range(int(inst.steps)):
                 L = tp.laser(i)
                 rows[i] = (ts[i], ts[i + 1], *L.pos0, *L.dir)
