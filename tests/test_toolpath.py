@@ -81,7 +81,7 @@ def test_run_toolpath_matches_oracle_and_stepper(cuda):
 @pytest.mark.gpu
 def test_batched_toolpath_equals_run_toolpath(cuda):
     """A toolpath instance in the batch engine (per-step source table) next to an ordinary
-    instance: bitwise the result of PFSolver.run_toolpath, for FP64 and FP32."""
+    instance: bitwise the result of PFSolver.run_toolpath, for FP64, FP32 and BF16 storage."""
     import torch
     from pfkit import PFSolver, workloads
     from pfkit.batch import BatchEngine, Instance
@@ -89,7 +89,7 @@ def test_batched_toolpath_equals_run_toolpath(cuda):
     n = len(tp) - 1
     ts = tp.schedule()[0]
     other = workloads.make("other", 20, 16, 12, 10, 12, 9, "float32", 0.15, seed=5, vel=2.0)
-    for dtype in ("float64", "float32"):
+    for dtype in ("float64", "float32", "bfloat16"):
         s = PFSolver(a, pc, dtype=dtype, device=cuda, clean=False)
         solo = s.run_toolpath(eta0, tp)
         eng = BatchEngine(max_instances=4)
