@@ -100,18 +100,20 @@ def candidates(pf_dtype, K, Nz):
 
 
 def time_candidates(solver_rhs, eta, T, t_prev, dt, reps=20, warmup=3):
-    """{(tile, zchunk): ms per step} for every candidate on this state."""
-    out = torch.empty_like(eta)
+    """{(tile, zchunk): ms per step} for every candidate: each runs warmup + reps consecutive
+    steps from this state (ping-pong buffers, time advancing by dt), as a solve does."""
     times = {}
     stream = torch.cuda.current_stream()
+    bufs = [torch.empty_like(eta), torch.empty_like(eta)]
     for cand in candidates(solver_rhs.pf_dtype, solver_rhs.K, int(solver_rhs.pf_args["Nz"])):
         tile, zc = cand
-        for _ in range(warmup):
-            solver_rhs.step(eta, out, T, t_prev, t_prev + dt, tile, zc)
+        bufs[0].copy_(eta)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            solver_rhs.step(eta, out, T, t_prev, t_prev + dt, tile, zc)
+        for i in range(warmup + reps):
+            if i == warmup:
+                e0.record(stream)
+            t = t_prev + i * dt
+            solver_rhs.step(bufs[i % 2], bufs[(i + 1) % 2], T, t, t + dt, tile, zc)
         e1.record(stream)
         e1.synchronize()
         times[cand] = e0.elapsed_time(e1) / reps
@@ -166,6 +168,8 @@ def _tune_one(dtype, K, aniso, Nz, Ny, Nx, states=(20, 400), log=None):
     k = key(s.state_rhs.pf_dtype, K, aniso, Nz, Ny, Nx)
     entry = dict(tile=best[0], zchunk=best[1], ms=round(times[best], 5), default_ms=round(times[(0, 0)], 5),
                  tuned_on=[Nz, Ny, Nx])
+    if tuple(states) != (20, 400):
+        entry["states"] = list(states)
     if log:
         log(f"{k}: tile {best[0]} zchunk {best[1]} {times[best]:.4f} ms (default {times[(0, 0)]:.4f} ms)")
     return k, entry
@@ -208,12 +212,30 @@ def _tune_batch(dtype, K, aniso, n_inst=12, warm=30, steps=60, seed=7, log=None)
     return k, entry
 
 
+# shapes tuned exactly: the c1/c2 grid in FP32, BF16 and FP64, isotropic FP32, and the c0 grid
+EXACT_JOBS = [("float32", 20, True, 64, 256, 256), ("bfloat16", 20, True, 64, 256, 256),
+              ("float64", 20, True, 64, 256, 256), ("float32", 20, False, 64, 256, 256),
+              ("float64", 10, False, 32, 64, 64)]
+
+
+def retune_exact(table_path, out_path, states=(20, 400, 800), log=None):
+    """Re-tune only the exact-shape entries of an existing table on the states reached after
+    each of ``states`` steps (a 1000-step run spends most of its steps in the developed states)
+    and write the merged table; the entries record the states they were tuned on."""
+    with open(table_path) as f:
+        doc = json.load(f)
+    for dtype, K, aniso, Nz, Ny, Nx in EXACT_JOBS:
+        k, e = _tune_one(dtype, K, aniso, Nz, Ny, Nx, states=states, log=log)
+        doc["entries"][k] = e
+    with open(out_path, "w") as f:
+        json.dump(doc, f, indent=1, sort_keys=True)
+    return doc
+
+
 def sweep(out_path, Ks=(10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20), log=None):
     """Tune the shipped table: c1/c2/FP64 shapes plus the batched/serving classes."""
     entries = {}
-    jobs = [("float32", 20, True, 64, 256, 256), ("bfloat16", 20, True, 64, 256, 256),
-            ("float64", 20, True, 64, 256, 256), ("float32", 20, False, 64, 256, 256),
-            ("float64", 10, False, 32, 64, 64)]
+    jobs = list(EXACT_JOBS)
     exact = len(jobs)
     for (Nz, Ny, Nx) in SWEEP_CLASSES:
         for K in Ks:
@@ -232,9 +254,9 @@ def sweep(out_path, Ks=(10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20), log=None):
             entries[k] = e
     doc = {"device": torch.cuda.get_device_name(0), "library_abi": int(_lib.load().pf_version()),
            "method": "single-instance entries: fastest of all (tile, zchunk) candidates by the mean time of 20 "
-                     "timed steps (after 3 warm-up steps) on the states reached after 20 and 400 steps of the "
-                     "workload's one-hot start; batch_* entries: fastest candidate for a batch of 12 instances "
-                     "advanced together by the batch engine",
+                     "consecutive ping-pong steps (after 3 warm-up steps) run from the states reached after 20 and "
+                     "400 steps of the workload's one-hot start; batch_* entries: fastest candidate for a batch "
+                     "of 12 instances advanced together by the batch engine",
            "entries": entries}
     with open(out_path, "w") as f:
         json.dump(doc, f, indent=1, sort_keys=True)
@@ -244,11 +266,17 @@ def sweep(out_path, Ks=(10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20), log=None):
 def main(argv=None):
     ap = argparse.ArgumentParser(description="tile autotuner of the fused step")
     ap.add_argument("--sweep", action="store_true", help="tune the shipped table's shape classes")
+    ap.add_argument("--exact", action="store_true",
+                    help="re-tune only the exact-shape entries of the shipped table (states after 20, 400, 800 steps)")
     ap.add_argument("--out", default=_TABLE_PATH)
     args = ap.parse_args(argv)
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     if args.sweep:
         torch.cuda.set_device(0)
-        sweep(args.out, log=lambda *a: print(*a, file=sys.stderr, flush=True))
+        sweep(args.out, log=log)
+    elif args.exact:
+        torch.cuda.set_device(0)
+        retune_exact(_TABLE_PATH, args.out, log=log)
 
 
 if __name__ == "__main__":
