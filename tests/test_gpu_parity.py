@@ -254,3 +254,25 @@ def test_reduce_is_deterministic_and_dtype_consistent(cuda):
     d1 = s.diagnostics(eta, L, 0.2)
     d2 = s.diagnostics(eta, L, 0.2)
     assert d1["F"] == d2["F"] and np.array_equal(d1["volumes"], d2["volumes"])
+
+
+def test_autotune_times_every_candidate_without_touching_the_state(cuda):
+    """autotune.tune times each (tile, zchunk) candidate over consecutive steps in buffers of its
+    own: every candidate gets a time, the choice is one of them, the caller's state is unchanged,
+    and the chosen shape steps bitwise like the generic kernel."""
+    from pfkit import autotune
+    Nx, Ny, Nz, K = 24, 16, 12, 8
+    a = args(Nx, Ny, Nz, K, anisotropy=0.15)
+    pc = poly(Nx, Ny, Nz, K)
+    L = gaussian(Nx, Ny, Nz)
+    s = _solver(a, pc, torch.float32, cuda)
+    eta = torch.as_tensor(random_eta(Nx * Ny * Nz, K, seed=9), dtype=torch.float32, device=cuda)
+    before = eta.clone()
+    best, times = autotune.tune(s.state_rhs, eta, L, 0.1, 1e-9, reps=2, warmup=1, store=False)
+    assert set(times) == set(autotune.candidates(s.state_rhs.pf_dtype, K, Nz))
+    assert best in times and all(t > 0.0 for t in times.values())
+    assert torch.equal(eta, before)
+    got, ref = torch.empty_like(eta), torch.empty_like(eta)
+    s.state_rhs.step(eta, got, L, 0.1, 0.1 + 1e-9, best[0], best[1])
+    s.state_rhs.step(eta, ref, L, 0.1, 0.1 + 1e-9, -2, 0)
+    assert torch.equal(got, ref)
