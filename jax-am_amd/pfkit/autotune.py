@@ -232,18 +232,38 @@ def retune_exact(table_path, out_path, states=(20, 400, 800), log=None):
     return doc
 
 
-def sweep(out_path, Ks=(10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20), log=None):
-    """Tune the shipped table: c1/c2/FP64 shapes plus the batched/serving classes."""
-    entries = {}
-    jobs = list(EXACT_JOBS)
-    exact = len(jobs)
+def _class_jobs(Ks):
+    """Size-class jobs: each class tuned on a grid of 3/4 of its upper bound per axis (at least 32)."""
+    jobs = []
     for (Nz, Ny, Nx) in SWEEP_CLASSES:
         for K in Ks:
             for dtype in ("float32", "bfloat16"):
-                jobs.append((dtype, K, True, Nz, Ny, Nx))
-    for j, (dtype, K, aniso, Nz, Ny, Nx) in enumerate(jobs):
-        if j >= exact:
-            Nz, Ny, Nx = (max(32, (3 * v) // 4) for v in (Nz, Ny, Nx))
+                jobs.append((dtype, K, True) + tuple(max(32, (3 * v) // 4) for v in (Nz, Ny, Nx)))
+    return jobs
+
+
+def retune_classes(table_path, out_path, Ks=(10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20), log=None):
+    """Re-tune only the size-class entries of an existing table (exact-shape and batch entries are
+    kept) and write the merged table."""
+    with open(table_path) as f:
+        doc = json.load(f)
+    exact = {key(_lib.TORCH_TO_PF[_dtype(d)], K, an, *dims) for d, K, an, *dims in EXACT_JOBS}
+    for dtype, K, aniso, Nz, Ny, Nx in _class_jobs(Ks):
+        k = key(_lib.TORCH_TO_PF[_dtype(dtype)], K, aniso, Nz, Ny, Nx)
+        if k in exact:
+            continue
+        k, e = _tune_one(dtype, K, aniso, Nz, Ny, Nx, log=log)
+        doc["entries"][k] = e
+    with open(out_path, "w") as f:
+        json.dump(doc, f, indent=1, sort_keys=True)
+    return doc
+
+
+def sweep(out_path, Ks=(10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20), log=None):
+    """Tune the shipped table: c1/c2/FP64 shapes plus the batched/serving classes."""
+    entries = {}
+    jobs = list(EXACT_JOBS) + _class_jobs(Ks)
+    for dtype, K, aniso, Nz, Ny, Nx in jobs:
         if key(_lib.TORCH_TO_PF[_dtype(dtype)], K, aniso, Nz, Ny, Nx) in entries:
             continue
         k, e = _tune_one(dtype, K, aniso, Nz, Ny, Nx, log=log)
@@ -268,6 +288,8 @@ def main(argv=None):
     ap.add_argument("--sweep", action="store_true", help="tune the shipped table's shape classes")
     ap.add_argument("--exact", action="store_true",
                     help="re-tune only the exact-shape entries of the shipped table (states after 20, 400, 800 steps)")
+    ap.add_argument("--classes", action="store_true",
+                    help="re-tune only the size-class entries of the shipped table")
     ap.add_argument("--out", default=_TABLE_PATH)
     args = ap.parse_args(argv)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
@@ -277,6 +299,9 @@ def main(argv=None):
     elif args.exact:
         torch.cuda.set_device(0)
         retune_exact(_TABLE_PATH, args.out, log=log)
+    elif args.classes:
+        torch.cuda.set_device(0)
+        retune_classes(_TABLE_PATH, args.out, log=log)
 
 
 if __name__ == "__main__":
