@@ -27,7 +27,10 @@
 extern "C" {
 #endif
 
-#define PFHIP_ABI_VERSION 1
+#define PFHIP_ABI_VERSION 2
+
+/* values per row of a batch instance's source table (pf_batch_inst_t.path_dev) */
+#define PF_PATH_ROW 10
 
 /* storage dtypes of eta (and of the optional temperature buffer) */
 enum pf_dtype { PF_F64 = 0, PF_F32 = 1, PF_BF16 = 2 };
@@ -191,6 +194,7 @@ int pf_reduce(const void* eta_dev, int32_t dtype, const void* T_dev, const int32
               const pf_params_t* p, double t, double* out_f64_dev, int64_t* out_i64_dev,
               void* ws_dev, size_t ws_bytes, void* stream);
 
+
 /*
  * Batched engine: many independent instances (uneven grids, K, dtype) advanced
  * together by persistent launches over a device-resident tile work list, captured
@@ -212,11 +216,13 @@ typedef struct pf_batch_inst {
   int32_t steps;              /* number of Euler steps to run */
   double t0, dt;              /* start time and fixed step */
   pf_params_t p;
-  /* Optional per-step source table (toolpaths, multi_scan/example.py:52-105), NULL for none:
-   * device array [steps][8] of FP64 rows (t_prev, t_crt, pos0 x, y, z, dir x, y, z).  Step i
-   * then runs from t_prev to t_crt of row i with p.laser.pos0/dir replaced by the row's, exactly
-   * as pf_step called with that row's times and source; t0/dt are then only the reported start.
-   * Needs a Rosenthal or Gaussian source.  The table is the caller's and must outlive the slot. */
+  /* Optional per-step source table, NULL for none: device array [steps][PF_PATH_ROW] of FP64
+   * rows (t_prev, t_crt, pos0 x, y, z, dir x, y, z, strength, vel), strength = Q (Rosenthal) or
+   * dT_peak (Gaussian).  Step i then runs from t_prev to t_crt of row i with p.laser's pos0, dir,
+   * strength and vel replaced by the row's, exactly as pf_step called with that row's times and
+   * source: toolpaths (multi_scan/example.py:52-105) and sources whose power and speed change
+   * with time (simple/example_grf.py:21-46).  t0/dt are then only the reported start.  Needs a
+   * Rosenthal or Gaussian source.  The table is the caller's and must outlive the slot. */
   const double* path_dev;
 } pf_batch_inst_t;
 

@@ -102,8 +102,10 @@ int make_stepdev(const pf_params_t* p, int dtype, const void* T_dev, const void*
   a.dirz = L.dir[2];
   a.vel = L.vel;
   if (L.model == PF_T_ROSENTHAL) {
-    a.q_coef = L.Q / (2.0 * M_PI * L.kappa_T);  // simple/example.py:35 Q / (2*pi*kappa)
-    a.pe = L.vel / (2.0 * L.alpha);             // vel / (2*alpha)
+    a.kappa_T = L.kappa_T;
+    a.alpha = L.alpha;
+    a.q_coef = rosenthal_q(L.Q, L.kappa_T);  // simple/example.py:35 Q / (2*pi*kappa)
+    a.pe = rosenthal_pe(L.vel, L.alpha);     // vel / (2*alpha)
   }
   a.T_cap = L.T_cap;
   a.dT_peak = L.dT_peak;
@@ -662,21 +664,35 @@ size_t pf_reduce_workspace_bytes(int64_t n_cells, int32_t k) {
   return (size_t)(nb * NF * 8 + nb * (k + NI) * 8 + nch * k * 8);
 }
 
-int pf_reduce(const void* eta_dev, int32_t dtype, const void* T_dev, const int32_t* argmax0_dev,
-              const pf_params_t* p, double t, double* out_f64_dev, int64_t* out_i64_dev, void* ws_dev,
-              size_t ws_bytes, void* stream) {
-  if (!eta_dev || !out_f64_dev || !out_i64_dev || !ws_dev) return fail(PF_E_ARG, "NULL buffer");
-  StepDev a;
+}  // extern "C"
+
+namespace {
+
+// Argument checks of pf_reduce (host only, no HIP call):
+// fills the instance description and the workspace size the reduction needs.
+int reduce_check(const void* eta_dev, int32_t dtype, const void* T_dev, const pf_params_t* p,
+                 const double* out_f64_dev, const int64_t* out_i64_dev, StepDev& a, size_t& need) {
+  if (!eta_dev || !out_f64_dev || !out_i64_dev) return fail(PF_E_ARG, "NULL buffer");
   int rc = make_stepdev(p, dtype, T_dev, (const void*)1, 0, a);
   if (rc) return rc;
-  const int K = p->num_oris;
-  if (K > RMAXK) return fail(PF_E_UNSUPPORTED, "pf_reduce supports K <= 256");
+  if (p->num_oris > RMAXK) return fail(PF_E_UNSUPPORTED, "pf_reduce supports K <= 256");
   const long long n = (long long)p->nz * p->ny * p->nx;
-  if (ws_bytes < pf_reduce_workspace_bytes(n, K)) return fail(PF_E_ARG, "workspace too small");
+  const long long nch = (n + kColRows - 1) / kColRows;
+  if (!grid_ok(nch * p->num_oris, 1)) return fail(PF_E_ARG, "grid too large");
+  need = pf_reduce_workspace_bytes(n, p->num_oris);
+  return PF_OK;
+}
+
+// The three reduction launches over a workspace of at least pf_reduce_workspace_bytes bytes.
+// Every partial result is written before it is read, so the workspace needs no clearing.
+int reduce_launch(const StepDev& a, const void* eta_dev, int32_t dtype, const int32_t* argmax0_dev,
+                  const pf_params_t* p, double t, double* out_f64_dev, int64_t* out_i64_dev, void* ws_dev,
+                  void* stream) {
+  const int K = p->num_oris;
+  const long long n = (long long)p->nz * p->ny * p->nx;
   const int nb = reduce_blocks(n);
   const long long per_block = (n + nb - 1) / nb;
   const long long nch = (n + kColRows - 1) / kColRows;
-  if (!grid_ok(nch * K, 1)) return fail(PF_E_ARG, "grid too large");
   double* part_f = (double*)ws_dev;
   long long* part_i = (long long*)(part_f + (long long)nb * NF);
   double* part_v = (double*)(part_i + (long long)nb * (K + NI));
@@ -704,6 +720,22 @@ int pf_reduce(const void* eta_dev, int32_t dtype, const void* T_dev, const int32
   reduce_final_kernel<<<1, 256, 0, s>>>(nb, nch, K, dV, part_f, part_i, part_v, out_f64_dev, (long long*)out_i64_dev);
   hipError_t err = hipGetLastError();
   return err == hipSuccess ? PF_OK : hip_fail(err, "pf_reduce launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+int pf_reduce(const void* eta_dev, int32_t dtype, const void* T_dev, const int32_t* argmax0_dev,
+              const pf_params_t* p, double t, double* out_f64_dev, int64_t* out_i64_dev, void* ws_dev,
+              size_t ws_bytes, void* stream) {
+  StepDev a;
+  size_t need = 0;
+  int rc = reduce_check(eta_dev, dtype, T_dev, p, out_f64_dev, out_i64_dev, a, need);
+  if (rc) return rc;
+  if (!ws_dev) return fail(PF_E_ARG, "NULL workspace");
+  if (ws_bytes < need) return fail(PF_E_ARG, "workspace too small");
+  return reduce_launch(a, eta_dev, dtype, argmax0_dev, p, t, out_f64_dev, out_i64_dev, ws_dev, stream);
 }
 
 }  // extern "C"

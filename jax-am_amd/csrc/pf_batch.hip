@@ -25,7 +25,7 @@
 namespace pf {
 
 // After each step: advance every active slot's counters and build its next step's arguments;
-// a toolpath slot also takes its next row's times and source position into its description.
+// a source-table slot also takes its next row's times and source into its description.
 __global__ __launch_bounds__(64) void advance_kernel(StepDev* __restrict__ descs, BatchSlot* __restrict__ slots,
                                                     StepDyn* __restrict__ dyns, int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -37,11 +37,11 @@ __global__ __launch_bounds__(64) void advance_kernel(StepDev* __restrict__ descs
     b.parity ^= 1;
     b.n += 1;
     if (b.path) {
-      b.tnow = b.path[8 * done + 1];
+      b.tnow = b.path[PF_PATH_ROW * done + 1];
       slots[s] = b;
       if (b.steps_left > 0) {
         StepDev a = descs[s];
-        const double* r = b.path + 8 * b.n;
+        const double* r = b.path + PF_PATH_ROW * b.n;
         apply_path_row(a, r);
         descs[s] = a;
         dyns[s] = make_dyn(a, b.eta[b.parity], b.eta[b.parity ^ 1], r[0], r[1] - r[0]);
@@ -60,6 +60,7 @@ struct BatchClass {
   int grid;
   int off;    // offset of this class's work list in d_work
   int count;  // host copy of the work-item count
+  int users;  // slots in this class; a class without users is retired and its index reused
 };
 
 struct SlotHost {
@@ -151,6 +152,21 @@ void drop_graphs(pf_batch* b) {
     (void)hipGraphDestroy(g.def);
   }
   b->graphs.clear();
+}
+
+// Destroy the cached graphs that launch class c (its index is about to be retired).  The
+// caller has synchronized the stream, so none of them is in flight.
+void drop_graphs_of_class(pf_batch* b, int c) {
+  const unsigned long long bit = 1ULL << c;
+  for (size_t i = 0; i < b->graphs.size();) {
+    if (b->graphs[i].mask & bit) {
+      (void)hipGraphExecDestroy(b->graphs[i].exec);
+      (void)hipGraphDestroy(b->graphs[i].def);
+      b->graphs.erase(b->graphs.begin() + i);
+    } else {
+      ++i;
+    }
+  }
 }
 
 // The graph of `chunk` steps for the current class set (captured on first use).
@@ -287,15 +303,27 @@ int pf_batch_add(pf_batch_t* b, const pf_batch_inst_t* inst) {
   for (const SlotHost& sh : b->slots)
     if (sh.used) total += sh.items;
   if (total + items > b->max_tiles) return fail(PF_E_STATE, "batch work list full (max_tiles)");
-  int cls = -1;
+  int cls = -1, free_cls = -1;
   for (size_t c = 0; c < b->classes.size(); ++c) {
     const BatchClass& k = b->classes[c];
-    if (k.dtype == inst->dtype && k.k == inst->p.num_oris && k.aniso == (int)aniso && k.tile == tile) cls = (int)c;
+    if (k.users == 0) {
+      if (free_cls < 0) free_cls = (int)c;
+    } else if (k.dtype == inst->dtype && k.k == inst->p.num_oris && k.aniso == (int)aniso && k.tile == tile) {
+      cls = (int)c;
+    }
   }
   if (cls < 0) {
-    if ((int)b->classes.size() >= pf_batch::kMaxClasses) return fail(PF_E_STATE, "too many instance classes (64)");
-    b->classes.push_back(BatchClass{inst->dtype, inst->p.num_oris, (int)aniso, tile, fn, grid, 0, 0});
-    cls = (int)b->classes.size() - 1;
+    // a new class takes a retired index (its graphs were dropped when it retired) or a new one
+    const BatchClass nc{inst->dtype, inst->p.num_oris, (int)aniso, tile, fn, grid, 0, 0, 0};
+    if (free_cls >= 0) {
+      b->classes[free_cls] = nc;
+      cls = free_cls;
+    } else {
+      if ((int)b->classes.size() >= pf_batch::kMaxClasses)
+        return fail(PF_E_STATE, "too many instance classes at once (64)");
+      b->classes.push_back(nc);
+      cls = (int)b->classes.size() - 1;
+    }
   }
   BatchSlot st;
   memset(&st, 0, sizeof(st));
@@ -312,7 +340,7 @@ int pf_batch_add(pf_batch_t* b, const pf_batch_inst_t* inst) {
   StepDyn dy = slot_dyn(a, st);
   hipError_t he = hipSuccess;
   if (inst->path_dev && inst->steps > 0) {  // the first row's times and source
-    double r0[8];
+    double r0[PF_PATH_ROW];
     he = hipMemcpyAsync(r0, inst->path_dev, sizeof(r0), hipMemcpyDeviceToHost, b->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(b->stream);
     if (he != hipSuccess) return hip_fail(he, "pf_batch_add path");
@@ -328,6 +356,7 @@ int pf_batch_add(pf_batch_t* b, const pf_batch_inst_t* inst) {
   b->slots[slot].used = true;
   b->slots[slot].cls = cls;
   b->slots[slot].items = items;
+  b->classes[cls].users += 1;
   b->work_dirty = true;
   return slot;
 }
@@ -341,7 +370,9 @@ int pf_batch_remove(pf_batch_t* b, int32_t slot) {
   hipError_t he = hipMemcpyAsync(b->d_slots + slot, &st, sizeof(BatchSlot), hipMemcpyHostToDevice, b->stream);
   if (he == hipSuccess) he = hipStreamSynchronize(b->stream);  // st is a local
   if (he != hipSuccess) return hip_fail(he, "pf_batch_remove");
+  const int cls = b->slots[slot].cls;
   b->slots[slot] = SlotHost();
+  if (--b->classes[cls].users == 0) drop_graphs_of_class(b, cls);  // retired: its index is free again
   b->work_dirty = true;
   return PF_OK;
 }

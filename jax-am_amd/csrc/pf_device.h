@@ -110,9 +110,15 @@ struct StepDev {
   // laser
   double T_amb, p0x, p0y, p0z, dirx, diry, dirz, vel;
   double q_coef, pe, T_cap;          // Rosenthal: Q/(2 pi kappa_T), vel/(2 alpha)
+  double kappa_T, alpha;             // Rosenthal: the source's conductivity and diffusivity
   double dT_peak, g_sxy, g_sz;       // Gaussian: 1/(2 sigma_xy^2), 1/(2 sigma_z^2)
   double T_ref, grad_T, cool_rate;   // quench
 };
+
+// Rosenthal coefficients Q / (2 pi kappa_T) and vel / (2 alpha) (simple/example.py:35), one
+// expression for the host and the device.
+__host__ __device__ inline double rosenthal_q(double Q, double kappa_T) { return Q / (2.0 * M_PI * kappa_T); }
+__host__ __device__ inline double rosenthal_pe(double vel, double alpha) { return vel / (2.0 * alpha); }
 
 // FP32 constants of one step: the temperature model at t and the cold-path mobility.
 struct ColdPath {
@@ -125,13 +131,85 @@ struct ColdPath {
   float T_amb, dT, gxy, gz, q, pe, dxr, dyr, dzr, T_cap, T_ref, grad_T, cool_t;
 };
 
-// One step of one instance: state buffers, time and the FP32 step constants.
+// One step of one instance: state buffers, time, the FP32 step constants and the step's
+// stochastic-rounding seed.
 struct StepDyn {
   const void* in;
   void* out;
   double t_prev, hstep;
   ColdPath cold;
+  uint32_t seed;
 };
+
+// ------------------------------------------------------------------ stochastic rounding
+// FP32 and BF16 storage round each Euler update stochastically instead of to nearest.  With
+// round-to-nearest, a value near eta = 1 whose per-step change lies below half an ulp (2^-25)
+// keeps its old value step after step while the FP64 solution moves on, so the error grows
+// linearly, by up to 2^-25 per step (measured on a 2.4 M-cell, 822-step solve: 2.3e-5 against
+// FP64, the same in a plain C FP32 restatement).  Rounding up with probability (distance to the
+// lower neighbour) / (gap) is unbiased, so the error only random-walks (same solve: 2.5e-6 after
+// 500 steps instead of 1.4e-5).  The random bits depend only on (cell index, channel, start time
+// of the step), so every kernel (tile shape, lane split, generic kernel, batch engine) produces
+// bitwise the same result.  The bits of channel i of a cell are key + i * 0x9e3779b9 (mod 2^32)
+// with key a murmur3-style hash of (cell, step): each channel's bits are uniform and independent
+// from step to step and from cell to cell, which is what unbiased rounding needs; within a cell
+// the channels' bits form a Weyl sequence (evenly spread, not independent).
+__host__ __device__ inline uint32_t mix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7feb352dU;
+  h ^= h >> 15;
+  h *= 0x846ca68bU;
+  h ^= h >> 16;
+  return h;
+}
+__host__ __device__ inline uint32_t step_seed(double t) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, t);
+  return mix32((uint32_t)b ^ mix32((uint32_t)(b >> 32) + 0x632be5abU));
+}
+// Key of cell (z, j) of a step, j = y * Nx + x its index within the z-plane (below 2^31): a hash
+// of the step seed and z (uniform across a plane, so the march forms it in scalar registers)
+// mixed with j.  Channel i of the cell takes key + i * kSrChannel.
+__host__ __device__ inline uint32_t sr_plane_key(uint32_t seed, int z) {
+  return mix32(seed ^ ((uint32_t)z * 0x85ebca6bU));
+}
+__device__ __forceinline__ uint32_t sr_cell_key(uint32_t plane_key, int j) { return mix32((uint32_t)j ^ plane_key); }
+constexpr uint32_t kSrChannel = 0x9e3779b9U;
+// c + d rounded stochastically to FP32.  s = RN(c + d) and its residual e = c + d - s (Fast2Sum:
+// exact when |c| >= |d|, which holds for every small update; a close approximation otherwise);
+// s moves by u = ulp(s) towards e with probability |e| / u, so the expected result is c + d.  (When s
+// is a power of two and e points to zero, the float below s lies u/2 away: the move then skips it,
+// which keeps the expectation exact.)  ulp(s) comes from the exponent bits of s (zero for s = 0 and
+// for subnormal s, where the result stays s).
+// The uniform variate is (h + bits of d) / 2^32: h is uniform and independent of d, so the sum
+// is too, and it cannot be formed before d exists.
+__device__ __forceinline__ float sr_add_f32(float c, float d, uint32_t h) {
+  const float s = c + d;
+  const float e = d - (s - c);
+  const float u = __uint_as_float(__float_as_uint(s) & 0x7f800000U) * 0x1p-23f;
+  const float thr = (float)(h + __float_as_uint(d)) * (u * 0x1p-32f);
+  return fabsf(e) > thr ? s + copysignf(u, e) : s;
+}
+// x rounded stochastically to the BF16 grid (returned as the FP32 value of that BF16): 16
+// random bits (the top half of h + bits of x, as in sr_add_f32) added below the BF16 mantissa,
+// then truncated.  Arithmetic NaNs are quiet (bit
+// 22 set), so they stay NaN; a carry past the largest finite value gives infinity.
+__device__ __forceinline__ float sr_bf16(float x, uint32_t h) {
+  const uint32_t xb = __float_as_uint(x);
+  return __uint_as_float((xb + ((h + xb) >> 16)) & 0xffff0000U);
+}
+// rounding mode of the Euler update by storage type: 0 none (FP64), 1 FP32, 2 BF16
+template <class TS> struct SrMode { static constexpr int value = 0; };
+template <> struct SrMode<float> { static constexpr int value = 1; };
+template <> struct SrMode<bf16_t> { static constexpr int value = 2; };
+// eta + increment d, stochastically rounded to the storage type TS (returned in FP32)
+template <class TS>
+__device__ __forceinline__ float sr_round(float c, float d, uint32_t h) {
+  if constexpr (SrMode<TS>::value == 1) {
+    return sr_add_f32(c, d, h);
+  } else {
+    return sr_bf16(c + d, h);
+  }
+}
 
 // Cold path: a cell whose FP32 mobility satisfies L(T) * |hstep * stiff| < 1e-6 changes by
 // less than 1e-6 * |eta| per step, so the relative error of its FP32 mobility (hardware
@@ -181,6 +259,7 @@ __host__ __device__ inline StepDyn make_dyn(const StepDev& a, const void* in, vo
   d.t_prev = t_prev;
   d.hstep = hstep;
   d.cold = make_cold(a, t_prev, hstep);
+  d.seed = step_seed(t_prev);
   return d;
 }
 
@@ -349,6 +428,13 @@ __device__ __forceinline__ CellCoef<TC> cell_coef(const StepDev& a, const ColdPa
   return cell_coef_with<TC>(a, k, T32, [&] { return cell_temperature(a, ix, iy, iz, c, t); }, hstep, S);
 }
 
+// A cell whose coefficients are not finite (non-finite S, or a non-finite temperature that makes
+// zeta and the mobility NaN): the reference's update of every one of its channels is then NaN,
+// a channel that is zero on its whole stencil included (0 * NaN in the local term), so such a
+// cell never takes the zero-channel skip.
+template <class TC>
+__device__ __forceinline__ bool coef_nonfinite(const CellCoef<TC>& cf) { return !isfinite(cf.A + cf.c1 + cf.c2); }
+
 // Bitwise "non-zero" test of a channel's 7-point stencil (a -0.0 counts as non-zero).
 __device__ __forceinline__ bool any_nz(float c, float a, float b, float d, float e, float f, float g) {
   return (__float_as_uint(c) | __float_as_uint(a) | __float_as_uint(b) | __float_as_uint(d) | __float_as_uint(e) |
@@ -366,6 +452,8 @@ __device__ __forceinline__ bool any_nz(double c, double a, double b, double d, d
 // The anisotropy factor is skipped when |grad eta|^2 == 0 on every lane of the wavefront:
 // there the full evaluation gives q = 1 and ca = w1 exactly, which is what the skip uses.
 // ax: the rotated cube axes of this orientation, axis k at ax[4k .. 4k+2].
+// With coefficients that carry dt (the step's), RHS = true returns the Euler increment
+// eta_new - eta, which the stochastically rounded stores add to eta (sr_round).
 template <class TC, bool ANISO, bool RHS = false>
 __device__ __forceinline__ TC update_one(const CellCoef<TC>& cf, TC B, TC ihx2, TC ihy2, TC ihz2,
                                          TC c, TC xm, TC xp, TC ym, TC yp, TC zm, TC zp,
@@ -787,6 +875,9 @@ struct March {
       const TC B = lap_B<TC>(a);
       TC ihx2, ihy2, ihz2;
       inv_h2<TC>(a, ihx2, ihy2, ihz2);
+      const bool bad = coef_nonfinite(cf);
+      constexpr bool SR = SrMode<TS>::value != 0;
+      uint32_t live = 0;  // wave-uniform: bit i set when channel i was updated, not skipped
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         const LVv vxm = *(const LVv*)(cb + oxm + g * LV);
@@ -798,14 +889,27 @@ struct March {
           const int i = g * LV + l;
           if (i < KH) {
             // A channel that is exactly zero on the whole stencil of every lane stays
-            // exactly +0 (the full update evaluates to +0 there); skip it wave-wide.
-            if (__ballot(any_nz(c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l], lo[i], hi[i])))
-              res[i] = update_one<TC, ANISO>(cf, B, ihx2, ihy2, ihz2, c[i], vxm.v[l], vxp.v[l], vym.v[l],
-                                             vyp.v[l], lo[i], hi[i], ax + 12 * (ch0 + i));
-            else
+            // exactly +0 (the full update evaluates to +0 there, finite coefficients given);
+            // skip it wave-wide.
+            if (__ballot(bad || any_nz(c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l], lo[i], hi[i]))) {
+              // FP32 / BF16 storage: the increment here, rounded stochastically onto eta below
+              res[i] = update_one<TC, ANISO, SR>(cf, B, ihx2, ihy2, ihz2, c[i], vxm.v[l], vxp.v[l], vym.v[l],
+                                                 vyp.v[l], lo[i], hi[i], ax + 12 * (ch0 + i));
+              live |= 1u << i;
+            } else {
               res[i] = (TC)0;
+            }
           }
         }
+      }
+      if constexpr (SR) {
+        // after the neighbour reads, so that the rounding's temporaries do not add to their registers
+        const uint32_t key = sr_cell_key(sr_plane_key(d.seed, z), gy * a.nx + gx) + (uint32_t)ch0 * kSrChannel;
+#pragma unroll
+        for (int i = 0; i < KH; ++i)
+          if (live & (1u << i)) res[i] = sr_round<TS>(c[i], res[i], key + (uint32_t)i * kSrChannel);
+      } else {
+        (void)live;
       }
       if (__ballot(cf.liquid)) {
 #pragma unroll
@@ -978,6 +1082,9 @@ __device__ __forceinline__ void generic_cell(const StepDev& a, const StepDyn& dy
   TC ihx2, ihy2, ihz2;
   inv_h2<TC>(a, ihx2, ihy2, ihz2);
   const TC* __restrict__ axes = (const TC*)a.axes;
+  const bool bad = coef_nonfinite(cf);
+  constexpr bool SR = !RHS && SrMode<TS>::value != 0;
+  const uint32_t key = sr_cell_key(sr_plane_key(dy.seed, iz), (int)(cell - (long long)iz * nx * ny));
   for (int i = 0; i < K; ++i) {
     TC ax[12];
     if constexpr (ANISO) {
@@ -990,10 +1097,12 @@ __device__ __forceinline__ void generic_cell(const StepDev& a, const StepDyn& dy
     const TC c = to_tc(in[e]), xm = to_tc(in[e + oxm]), xp = to_tc(in[e + oxp]), ym = to_tc(in[e + oym]),
              yp = to_tc(in[e + oyp]), zm = to_tc(in[e + ozm]), zp = to_tc(in[e + ozp]);
     TC v;
-    if (RHS || __ballot(any_nz(c, xm, xp, ym, yp, zm, zp)))
-      v = update_one<TC, ANISO, RHS>(cf, B, ihx2, ihy2, ihz2, c, xm, xp, ym, yp, zm, zp, ax);
-    else
+    if (RHS || __ballot(bad || any_nz(c, xm, xp, ym, yp, zm, zp))) {
+      v = update_one<TC, ANISO, RHS || SR>(cf, B, ihx2, ihy2, ihz2, c, xm, xp, ym, yp, zm, zp, ax);
+      if constexpr (SR) v = sr_round<TS>(c, v, key + (uint32_t)i * kSrChannel);
+    } else {
       v = (TC)0;
+    }
     if (!RHS && cf.liquid) v = (TC)0;
     out[e] = from_tc<TS>(v);
   }

@@ -18,11 +18,13 @@ struct BatchSlot {
   double t0, dt;
   void* eta[2];
   double tnow;         // time of the current state
-  const double* path;  // per-step rows (t_prev, t_crt, pos0 xyz, dir xyz) of a toolpath, or null
+  const double* path;  // per-step rows (PF_PATH_ROW values, include/pfhip.h) of a source table, or null
 };
 
-// A toolpath row's source position and direction into an instance description (the fields
-// make_stepdev copies from pf_laser_t pos0/dir).
+// A source-table row's position, direction, strength and speed into an instance description:
+// the fields make_stepdev derives from pf_laser_t pos0, dir, Q (Rosenthal) or dT_peak
+// (Gaussian) and vel, with the same expressions, so a row gives bitwise the description that
+// pf_step builds from a descriptor holding those values.
 __host__ __device__ inline void apply_path_row(StepDev& a, const double* r) {
   a.p0x = r[2];
   a.p0y = r[3];
@@ -30,6 +32,13 @@ __host__ __device__ inline void apply_path_row(StepDev& a, const double* r) {
   a.dirx = r[5];
   a.diry = r[6];
   a.dirz = r[7];
+  a.vel = r[9];
+  if (a.tmodel == PF_T_ROSENTHAL) {
+    a.q_coef = rosenthal_q(r[8], a.kappa_T);
+    a.pe = rosenthal_pe(r[9], a.alpha);
+  } else {
+    a.dT_peak = r[8];
+  }
 }
 
 struct WorkItem {

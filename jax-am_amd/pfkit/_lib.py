@@ -15,13 +15,22 @@ LIB_PATH = os.environ.get("PFKIT_LIB") or os.path.join(_HERE, "libpfhip.so")
 
 PF_F64, PF_F32, PF_BF16 = 0, 1, 2
 PF_T_BUFFER, PF_T_ROSENTHAL, PF_T_GAUSSIAN, PF_T_QUENCH = 0, 1, 2, 3
+PF_PATH_ROW = 10                  # values per row of a batch instance's source table
+ABI_VERSION = 2
 
 TORCH_TO_PF = {torch.float64: PF_F64, torch.float32: PF_F32, torch.bfloat16: PF_BF16}
 PF_TO_TORCH = {v: k for k, v in TORCH_TO_PF.items()}
 
 
+PF_OK, PF_E_ARG, PF_E_UNSUPPORTED, PF_E_HIP, PF_E_STATE = 0, -1, -2, -3, -4
+
+
 class PFError(RuntimeError):
-    """A libpfhip.so call returned an error code."""
+    """A libpfhip.so call returned an error code (``code``: one of PF_E_*)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 class LaserT(ctypes.Structure):
@@ -129,7 +138,7 @@ def check(rc, what=""):
     """Raise PFError with the library's message when rc is a PF_E* code."""
     if rc != 0:
         msg = load().pf_last_error().decode(errors="replace")
-        raise PFError(f"{what}: {msg} (code {rc})" if what else f"{msg} (code {rc})")
+        raise PFError(f"{what}: {msg} (code {rc})" if what else f"{msg} (code {rc})", code=rc)
     return rc
 
 

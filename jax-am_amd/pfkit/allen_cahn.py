@@ -197,10 +197,12 @@ class PFSolver:
         return a
 
     def run_toolpath(self, eta0, toolpath, steps=None, callback=None):
-        """Advance along a ``pfkit.laser.Toolpath`` (multi_scan/example.py:88-105): step i goes from
-        ts[i] to ts[i+1] with the source of ``toolpath.laser(i)``.  ``callback(i + 1, eta)`` runs after
-        each step (eta is the live buffer; copy it to keep it).  Returns the final eta."""
-        ts = toolpath.schedule()[0]
+        """Advance along a source table: a ``pfkit.laser.Toolpath`` (multi_scan/example.py:88-105) or
+        ``RosenthalRamp`` (simple/example_grf.py:21-67).  Step i goes from ts[i] to ts[i+1]
+        (ts = ``toolpath.step_times()``) with the source of ``toolpath.laser(i)``.
+        ``callback(i + 1, eta)`` runs after each step (eta is the live buffer; copy it to keep it).
+        Returns the final eta."""
+        ts = toolpath.step_times()
         n = len(ts) - 1 if steps is None else min(int(steps), len(ts) - 1)
         a = self.state_rhs.check_state(eta0).clone()
         b = torch.empty_like(a)

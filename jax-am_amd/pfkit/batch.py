@@ -18,6 +18,7 @@ import torch
 from . import _lib
 from .allen_cahn import DTYPES
 from .config import make_params, with_defaults
+from .laser import Rosenthal
 
 
 @dataclass
@@ -37,12 +38,14 @@ class Instance:
     tile: object = None             # None: the autotuner's throughput choice (pfkit.autotune.lookup)
     zchunk: object = None
     tag: object = None
-    toolpath: object = None         # pfkit.laser.Toolpath: step i runs ts[i] -> ts[i+1] with its laser(i)
+    toolpath: object = None         # source table (pfkit.laser.Toolpath, RosenthalRamp): step i runs
+                                    # step_times()[i] -> [i+1] with its laser(i)
     # filled by the engine
     slot: int = -1
     bufs: list = field(default_factory=list)
     axes: object = None
     path: object = None             # device table of the toolpath's per-step rows
+    params: object = None           # the pf_params_t the slot runs with
 
     @property
     def N(self):
@@ -70,6 +73,10 @@ class BatchEngine:
             raise _lib.PFError("pf_batch_create: " + self.lib.pf_last_error().decode())
         self.max_instances = max_instances
         self.insts = {}
+        # outputs of the retirement check (pf_reduce), allocated once
+        self._red_f = torch.empty(4 + 256, dtype=torch.float64, device=self.device)
+        self._red_i = torch.empty(256 + 3, dtype=torch.int64, device=self.device)
+        self._red_ws = None
 
     # ---- instance lifecycle
     def add(self, inst: Instance) -> int:
@@ -110,17 +117,24 @@ class BatchEngine:
         bi.dt = float(inst.dt)
         laser, path = inst.laser, None
         if inst.toolpath is not None:
-            # rows (t_prev, t_crt, pos0, dir) of the steps, as PFSolver.run_toolpath passes them to pf_step
+            # rows (t_prev, t_crt, pos0, dir, strength, vel) of the steps, as PFSolver.run_toolpath
+            # passes them to pf_step (PF_PATH_ROW values; strength = Q or dT_peak)
             tp = inst.toolpath
             if getattr(tp, "apply_switch", False):
                 raise ValueError("batched toolpaths run with the source always on (apply_switch=False)")
-            ts = tp.schedule()[0]
+            ts = tp.step_times()
             if inst.steps > len(ts) - 1:
                 raise ValueError(f"the toolpath has {len(ts) - 1} steps, {inst.steps} asked for")
-            rows = np.zeros((max(int(inst.steps), 1), 8))
+            rows = np.zeros((max(int(inst.steps), 1), _lib.PF_PATH_ROW))
+            kind = None
             for i in range(int(inst.steps)):
                 L = tp.laser(i)
-                rows[i] = (ts[i], ts[i + 1], *L.pos0, *L.dir)
+                if kind is None:
+                    kind = type(L)
+                elif type(L) is not kind:
+                    raise ValueError("every step of a batched source table needs the same source model")
+                strength = L.Q if isinstance(L, Rosenthal) else L.dT_peak
+                rows[i] = (ts[i], ts[i + 1], *L.pos0, *L.dir, strength, L.vel)
             path = torch.as_tensor(rows, dtype=torch.float64, device=self.device)
             laser = tp.laser(0)
             bi.t0 = float(ts[0])
@@ -128,6 +142,7 @@ class BatchEngine:
         else:
             bi.path_dev = None
         bi.p = make_params(a, inst.h_xyz, laser)
+        inst.params = bi.p
         self.stream.wait_stream(torch.cuda.current_stream(self.device))  # the table is copied on the caller's stream
         slot = self.lib.pf_batch_add(self.h, ctypes.byref(bi))
         if slot < 0:
@@ -153,6 +168,27 @@ class BatchEngine:
         pa = (ctypes.c_int32 * n)()
         _lib.check(self.lib.pf_batch_status(self.h, sl, tt, pa, n), "pf_batch_status")
         return np.frombuffer(sl, dtype=np.int32).copy(), np.frombuffer(tt).copy(), np.frombuffer(pa, np.int32).copy()
+
+    def nonfinite(self, slot, parity=None):
+        """Number of inf/nan values in slot's current state (FP64 reduction on the engine's stream)."""
+        inst = self.insts[slot]
+        eta = self.state(slot, parity)
+        p = inst.params
+        lib = self.lib
+        st = self.stream.cuda_stream
+        if inst.K > 256:
+            with torch.cuda.stream(self.stream):
+                return int((~torch.isfinite(eta)).sum().item())
+        Tptr = None  # batch instances evaluate a source descriptor, never a temperature buffer
+        need = lib.pf_reduce_workspace_bytes(inst.N, inst.K)
+        if self._red_ws is None or self._red_ws.numel() < need:
+            with torch.cuda.stream(self.stream):
+                self._red_ws = torch.empty(need, dtype=torch.uint8, device=self.device)
+        rc = lib.pf_reduce(eta.data_ptr(), _lib.TORCH_TO_PF[eta.dtype], Tptr, None, p, float(inst.t0),
+                           self._red_f.data_ptr(), self._red_i.data_ptr(), self._red_ws.data_ptr(), need, st)
+        _lib.check(rc, "nonfinite check")
+        self.stream.synchronize()
+        return int(self._red_i[inst.K + 2].item())
 
     def state(self, slot, parity=None):
         """The device tensor holding slot's current state."""

@@ -160,6 +160,9 @@ class Toolpath:
     def __len__(self):
         return len(self.schedule()[0])
 
+    def step_times(self):
+        return self.schedule()[0]
+
     def laser(self, i):
         """The temperature descriptor of step i (valid at t = ts[i])."""
         ts, xs, ys, zs, ps, dirs = self.schedule()
@@ -172,3 +175,51 @@ class Toolpath:
                          kappa_T=self.kappa_T, T_ambient=self.T_ambient, dir=tuple(float(v) for v in d),
                          T_cap=self.T_cap)
 
+
+
+@dataclass
+class RosenthalRamp:
+    """Rosenthal source whose power and speed ramp linearly in time, simple/example_grf.py:21-46:
+    Q(t) = Q_max + (Q_min - Q_max) t / t_OFF, vel(t) = vel_min + (vel_max - vel_min) t / t_OFF, and
+    the source at x0 + (vel(t) + vel_min) t / 2 (the integral of vel), y0, z0, moving along +x;
+    the exponent uses vel(t).  ``laser(i)`` is the Rosenthal descriptor of step i, valid at
+    t = ts[i] (pf_step evaluates T at t_prev, as the reference's loop does, example_grf.py:61-63),
+    with ts = numpy.arange(0, t_OFF + 1e-10, dt) (example_grf.py:57); the batch engine takes it as a
+    source table (pf_batch_inst_t.path_dev)."""
+    x0: float
+    y0: float
+    z0: float
+    t_OFF: float
+    dt: float
+    vel_avg: float
+    Q_max: float = 35.0
+    Q_min: float = 15.0
+    alpha: float = 5.2e-6
+    kappa_T: float = 27.0
+    T_ambient: float = 300.0
+
+    @classmethod
+    def grf_app(cls, pf_args):
+        """example_grf.py:26-33: x0 = 0.2 Lx, y = Ly/2, z = Lz, vel_avg = 0.6 Lx / t_OFF."""
+        return cls(x0=0.2 * pf_args["domain_x"], y0=0.5 * pf_args["domain_y"], z0=pf_args["domain_z"],
+                   t_OFF=pf_args["t_OFF"], dt=pf_args["dt"], vel_avg=0.6 * pf_args["domain_x"] / pf_args["t_OFF"],
+                   T_ambient=pf_args.get("T_ambient", 300.0))
+
+    def step_times(self):
+        import numpy as np
+        return np.arange(0., self.t_OFF + 1e-10, self.dt)
+
+    def __len__(self):
+        return len(self.step_times())
+
+    def at(self, t):
+        """The descriptor of time t: Q(t), vel(t) and pos0 such that pos0 + vel(t) t = x0 + (vel(t) + vel_min) t / 2."""
+        vel_min, vel_max = 0.5 * self.vel_avg, 1.5 * self.vel_avg
+        Q = t / self.t_OFF * (self.Q_min - self.Q_max) + self.Q_max
+        vel = t / self.t_OFF * (vel_max - vel_min) + vel_min
+        x0 = self.x0 + (vel + vel_min) / 2. * t - vel * t
+        return Rosenthal(pos0=(x0, self.y0, self.z0), vel=vel, Q=Q, alpha=self.alpha, kappa_T=self.kappa_T,
+                         T_ambient=self.T_ambient)
+
+    def laser(self, i):
+        return self.at(float(self.step_times()[i]))

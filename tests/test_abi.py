@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_version_and_struct_sizes():
     lib = _lib.load()
-    assert lib.pf_version() == 1
+    assert lib.pf_version() == _lib.ABI_VERSION == 2
     sp, sl, sb = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
     lib.pf_abi_sizes(ctypes.byref(sp), ctypes.byref(sl), ctypes.byref(sb))
     assert (sp.value, sl.value, sb.value) == (ctypes.sizeof(_lib.ParamsT), ctypes.sizeof(_lib.LaserT),
@@ -81,3 +81,4 @@ def test_argument_validation_without_gpu():
     big = make_params(args(8192, 8192, 1, 20), (1.0, 1.0, 1.0))
     rc = lib.pf_step(16, 32, _lib.PF_F32, 3, big, None, 0.0, 1.0, 0, 0, None)
     assert rc == -1 and b"2^32 bytes" in lib.pf_last_error()
+

@@ -4,7 +4,7 @@ The CPU tests re-run the FP64 oracle on each fixture's inputs and require its st
 output, so any change of the oracle shows up here.  The GPU tests run the HIP path through
 the drop-in solver on the same inputs: FP64 within 1e-12 relative (max norm) per step and
 after the whole fixture trajectory, FP32 within 2e-5 absolute, BF16 storage within the
-BF16 rounding of each step plus 2e-2 absolute after the trajectory, and the FP64
+BF16 rounding of each step (stochastic: one BF16 ulp) plus 2e-2 absolute after the trajectory, and the FP64
 diagnostics (free energy, orientation volumes) within 1e-12 relative.
 """
 import glob
@@ -98,6 +98,7 @@ def test_gpu_f32_and_bf16_match_fixture(path, cuda):
                  g["grain_dirs"] if g["pf_args"]["anisotropy"] > 0 else None, laser=g["laser_dict"],
                  cent=O.centroids(g["pf_args"]["Nz"], g["pf_args"]["Ny"], g["pf_args"]["Nx"],
                                   tuple(g["h_xyz"])))
-    assert np.all(np.abs(got1 - ref1) <= 2.0 ** -8 * np.abs(ref1) + 2e-6)
+    # stochastic rounding to BF16 lands on one of the two BF16 neighbours: within 1 ulp (2^-7 relative)
+    assert np.all(np.abs(got1 - ref1) <= 2.0 ** -7 * np.abs(ref1) + 2e-6)
     gotN = s16.run(e0, g["ts"], g["laser_obj"]).double().cpu().numpy()
     assert np.max(np.abs(gotN - g["eta_final"])) <= 2e-2 + 2.0 ** -7

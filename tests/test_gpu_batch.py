@@ -88,3 +88,27 @@ def test_server_results_same_alone_and_in_the_mix(cuda):
         assert r.done is not None and r.latency >= 0
         assert torch.equal(r.result, ref), r.rid
     eng.close()
+
+
+def test_retired_classes_free_their_index(cuda):
+    # 72 distinct (dtype, K) classes one after another: more than the engine's 64 class indices,
+    # so retired classes must give their index (and their cached graphs) back; every instance
+    # still runs the arithmetic of its own solo run
+    eng = BatchEngine(max_instances=4)
+    keep = workloads.make("keep", 12, 10, 6, 7, 4, 400, "float32", 0.15, seed=90, vel=2.0)
+    ks = eng.add(keep.instance())
+    n = 0
+    for dtype in ("float32", "bfloat16", "float64"):
+        for k in range(1, 25):
+            w = workloads.make(f"cls{dtype}{k}", 10, 8, 4, k, 4, 3, dtype, 0.15, seed=k, vel=2.0)
+            sl = eng.add(w.instance())
+            eng.run(3, 3)
+            _, _, par = eng.status()
+            assert torch.equal(eng.state(sl, par[sl]), _solo(w)), w.name
+            eng.remove(sl)
+            n += 1
+    assert n == 72
+    eng.run_to_completion(chunk=8)
+    _, _, par = eng.status()
+    assert torch.equal(eng.state(ks, par[ks]), _solo(keep))
+    eng.close()

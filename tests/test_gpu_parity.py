@@ -2,9 +2,11 @@
 
 Tolerances (written here, checked per test):
   F64 storage/compute: relative L-inf <= 1e-12 per step (max|d| / max|eta|).
-  F32 storage/compute: |d| <= 2e-6 * max|eta| + 1e-7 per step.
-  BF16 storage, F32 compute: the only extra error is the final round-to-nearest-even to
-  BF16, so |d| <= 2^-8 |eta_new| + 2e-6 (one bf16 half-ulp plus FP32 compute error).
+  F32 storage/compute: |d| <= 2e-6 * max|eta| + 1e-7 per step (stochastic rounding: within one
+  FP32 ulp of the FP32 arithmetic's result).
+  BF16 storage, F32 compute: the only extra error is the final stochastic rounding to BF16, which
+  lands on one of the two BF16 neighbours, so |d| <= 2^-7 |eta_new| + 2e-6 (one bf16 ulp plus FP32
+  compute error).
 """
 import numpy as np
 import pytest
@@ -104,7 +106,7 @@ def test_bf16_step_matches_oracle(case, cuda):
     s = _solver(a, pc, torch.bfloat16, cuda)
     got = _gpu_step(s, eta, L, 0.3, 0.32, torch.bfloat16).to(torch.float64).cpu().numpy()
     ref = _oracle_step(eta, a, pc, L, 0.3, 0.32)
-    assert np.all(np.abs(got - ref) <= 2.0 ** -8 * np.abs(ref) + 2e-6)
+    assert np.all(np.abs(got - ref) <= 2.0 ** -7 * np.abs(ref) + 2e-6)
 
 
 # (37, 21, 19): partial tiles in x and y; (40, 64, 9): tile rows divisible by 8, which selects
