@@ -13,8 +13,9 @@ MI355X HBM (8 TB/s) with 8 algorithmic bytes per FP32 update (read + write eta),
 averaging the kernel's duration with HIP events recorded on the stream the kernel runs
 on.  ``cpu_baseline`` times the C/OpenMP FP64 restatement of the reference step on
 this host's cores for a bounded number of steps of the same grid.  Extra keys report
-BF16 storage, the FP64 mode, the 1000-step error of FP32/BF16 against the FP64 mode,
-and (unless --quick) the batched and serving paths.
+BF16 storage, the FP64 mode, a dense-state step, the 1000-step error of FP32/BF16 against
+the FP64 mode (run first: it also brings the GPU to its working clocks before the timed
+steps), more CPU legs, and (unless --quick) the batched and serving paths.
 """
 import argparse
 import json
@@ -62,13 +63,82 @@ def time_steps(solver, eta0, laser, ts, warmup, steps, tile=None, zchunk=None):
     return wall, e0.elapsed_time(e1), a
 
 
-def load_traffic(name):
+def load_traffic(name, tile, zchunk):
+    """HBM bytes per launch from the rocprofv3 counter passes summarised in profiles/, or None when
+    they were measured on another tile shape or z-chunk than the one timed here."""
     path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     try:
         with open(path) as f:
-            return json.load(f)
+            tr = json.load(f)
     except (OSError, ValueError):
         return None
+    if tr.get("tile") != tile or tr.get("zchunk") != zchunk:
+        return None
+    return tr
+
+
+def cpu_legs(wl, seconds):
+    """More CPU baselines (reported beside cpu_baseline): the NumPy FP64 oracle on the whole
+    64x64x32, K = 10, 200-step CPU case (single process), and the C/OpenMP restatement in FP32 on
+    the timed grid for ~`seconds`."""
+    import numpy as np
+    from oracle import c_oracle as C
+    from oracle import pf_oracle as O
+    from pfkit import workloads
+    out = {}
+    w0 = workloads.c0()
+    P, h = w0.pf_args, w0.polycrystal.mesh_h_xyz
+    cent = O.centroids(P["Nz"], P["Ny"], P["Nx"], h)
+    eta = O.ini_cond(w0.polycrystal.cell_ori_inds, w0.K)
+    ts = w0.ts()
+    t0 = time.perf_counter()
+    eta = O.run(eta, ts, P, h, None, laser=w0.laser.as_dict(), cent=cent)
+    el = time.perf_counter() - t0
+    out["c0_numpy_fp64"] = dict(value=w0.N * w0.K * w0.steps / el, unit="cell*orientation updates/s", cores=1,
+                                kind="port", seconds=el,
+                                sample=f"64x64x32 grid, K=10, {w0.steps} steps (the whole run), FP64, "
+                                       "oracle/pf_oracle.py (NumPy)")
+    if C.available():
+        cores = len(os.sched_getaffinity(0))
+        params = C.make_params(wl.pf_args, wl.polycrystal.mesh_h_xyz, wl.laser.as_dict())
+        e = np.zeros((wl.N, wl.K), dtype=np.float32)
+        e[np.arange(wl.N), wl.polycrystal.cell_ori_inds] = 1.0
+        o = np.empty_like(e)
+        axes = wl.polycrystal.unique_grain_directions
+        ts = wl.ts(10000)
+        C.step(e, o, params, axes, ts[0], ts[1], threads=cores)
+        e, o = o, e
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            C.step(e, o, params, axes, ts[n + 1], ts[n + 2], threads=cores)
+            e, o = o, e
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= seconds or n >= 200:
+                break
+        out["c1_c_fp32"] = dict(value=wl.N * wl.K * n / el, unit="cell*orientation updates/s", cores=cores,
+                                kind="port", sample=f"{wl.polycrystal.Nz}x{wl.polycrystal.Ny}x{wl.polycrystal.Nx}"
+                                                    f"x{wl.K} grid, FP32, {n} steps after 1 warm-up step, {el:.1f} s, "
+                                                    f"OpenMP over {cores} threads (oracle/pf_oracle.c)")
+    return out
+
+
+def dense_state(wl, steps, warmup):
+    """The FP32 step on a dense state (eta ~ U[0, 1], seeded), where no channel is zero on its
+    whole stencil, so the zero-channel skip never applies."""
+    import torch
+    from pfkit import PFSolver
+    s = PFSolver(wl.pf_args, wl.polycrystal, dtype="float32", clean=False)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    eta0 = torch.rand((wl.N, wl.K), generator=g, device="cuda", dtype=torch.float32)
+    ts = wl.ts(warmup + steps)
+    w, e, _ = time_steps(s, eta0, wl.laser, ts, warmup, steps)
+    k = e / 1e3 / steps
+    return dict(ms_per_step=w / steps * 1e3, kernel_ms=k * 1e3, value=wl.N * wl.K * steps / w,
+                roofline_frac=BYTES_PER_UPDATE["float32"] * wl.N * wl.K / k / 1e9 / HBM_PEAK_GBS,
+                steps=steps, warmup=warmup, tile=s.tile, zchunk=s.zchunk,
+                state="eta ~ U[0,1] (torch.rand, seed 7), FP32")
 
 
 def cpu_baseline(wl, seconds):
@@ -182,6 +252,12 @@ def main():
                           "roofline_frac": BYTES_PER_UPDATE[dt] * wl.N * wl.K / (e / 1e3 / args.steps) / 1e9
                           / HBM_PEAK_GBS, "tile": so.tile, "zchunk": so.zchunk}))
         return
+    acc = None
+    if not args.only_step and not args.quick:
+        # the 1000-step accuracy runs (FP64, FP32, BF16) come first: besides their own result they
+        # bring the GPU to its working clocks before the timed steps
+        log(f"accuracy after {wl.steps} steps vs the FP64 mode...")
+        acc = accuracy(wl, wl.steps)
     s = PFSolver(wl.pf_args, wl.polycrystal, dtype="float32", clean=False, tile=args.tile, zchunk=args.zchunk)
     eta0 = s.ini_cond()
     ts = wl.ts(args.warmup + args.steps)
@@ -191,7 +267,7 @@ def main():
     value = upd * args.steps / wall
     kern_s = ev_ms / 1e3 / args.steps
     achieved = BYTES_PER_UPDATE["float32"] * upd / kern_s / 1e9
-    tr = load_traffic("c1_f32")
+    tr = load_traffic("c1_f32", s.tile, s.zchunk)
     out = {
         "metric": "cell*orientation updates/s (fused Allen-Cahn explicit step)",
         "value": value,
@@ -206,6 +282,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (tr or {}).get("bytes_per_launch"),
+                     "traffic_measured_at": None if tr is None else {"tile": tr["tile"], "zchunk": tr["zchunk"],
+                                                                     "summary": tr.get("summary")},
                      "kernel_ms": kern_s * 1e3,
                      "algorithmic_bytes_per_launch": BYTES_PER_UPDATE["float32"] * upd},
     }
@@ -223,11 +301,18 @@ def main():
         w3, e3, _ = time_steps(s3, s3.ini_cond(), wl.laser, ts, 3, n3)
         out["f64"] = {"value": upd * n3 / w3, "ms_per_step": w3 / n3 * 1e3,
                       "roofline_frac": BYTES_PER_UPDATE["float64"] * upd / (e3 / 1e3 / n3) / 1e9 / HBM_PEAK_GBS}
-        log(f"accuracy after {wl.steps} steps vs the FP64 mode...")
-        out["accuracy_vs_f64"] = accuracy(wl, wl.steps)
+        log("FP32 step on a dense state...")
+        out["dense_state"] = dense_state(wl, max(20, args.steps // 10), 5)
+        if acc is None:
+            log(f"accuracy after {wl.steps} steps vs the FP64 mode...")
+            acc = accuracy(wl, wl.steps)
+        out["accuracy_vs_f64"] = acc
     if not args.only_step:
         log(f"CPU baseline (~{args.cpu_seconds:.0f} s)...")
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+        if not args.quick:
+            log("more CPU legs (NumPy FP64 on the 64x64x32 case, C FP32 on this grid)...")
+            out["cpu_legs"] = cpu_legs(wl, args.cpu_seconds / 2)
     if not args.quick and not args.only_step:
         try:
             from pfkit import batchbench

@@ -22,6 +22,10 @@ def run_batched(log, n=64, chunk=20, seed=3):
         s = PFSolver(w.pf_args, w.polycrystal, dtype="float32", clean=False)
         solvers.append(s)
         eta0s.append(s.ini_cond())
+    # untimed: one step of each solver, so that every kernel (each K and tile shape) is loaded
+    # before the timed loop, as the batched leg's warm-up does for its kernels
+    for s, e, w in zip(solvers, eta0s, insts):
+        s.run(e, w.ts(1), w.laser)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     alone = [s.run(e, w.ts(), w.laser) for s, e, w in zip(solvers, eta0s, insts)]
@@ -52,7 +56,7 @@ def run_batched(log, n=64, chunk=20, seed=3):
                 classes=sorted({(w.K, w.dtype) for w in insts}))
 
 
-def run_serving(log, rates=(4.0, 8.0, 16.0, 24.0, 32.0), n_per_level=24, chunk=16, budget_gb=96.0,
+def run_serving(log, rates=(4.0, 8.0, 16.0, 24.0, 32.0), n_per_level=24, chunk=16, budget_gb=2.0,
                 sample_every=8):
     """c4: for each arrival rate, Poisson arrivals of n_per_level requests served with
     continuous batching; reports throughput and latency percentiles, and the FP64-mode error
@@ -70,9 +74,12 @@ def run_serving(log, rates=(4.0, 8.0, 16.0, 24.0, 32.0), n_per_level=24, chunk=1
                      keep_results=True)
         srv.serve(reqs)
         st = summarize(reqs, lambda r: r.instance.N * r.instance.K * r.instance.steps)
-        st.update(rate_per_s=lam, deferred_admissions=srv.deferred_for_memory)
+        st.update(rate_per_s=lam, hbm_budget_gb=budget_gb, deferred_admissions=srv.deferred_for_memory,
+                  deferred_for_capacity=srv.deferred_for_capacity)
         errs = []
         for r in reqs[::sample_every]:
+            if not r.ok:
+                continue
             w = r.meta["wl"]
             s64 = PFSolver(w.pf_args, w.polycrystal, dtype="float64", clean=False)
             ref = s64.run(s64.ini_cond(), w.ts(), w.laser)
