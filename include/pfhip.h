@@ -194,6 +194,21 @@ int pf_reduce(const void* eta_dev, int32_t dtype, const void* T_dev, const int32
               const pf_params_t* p, double t, double* out_f64_dev, int64_t* out_i64_dev,
               void* ws_dev, size_t ws_bytes, void* stream);
 
+/*
+ * pf_reduce without a workspace: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results as pf_reduce minus ws_dev/ws_bytes; the
+ * partial results live in a fixed device array of PF_REDUCE_SCRATCH_BYTES bytes inside the
+ * library, one per GPU, held until the process ends (its device address is looked up on the
+ * first call on each device).  Calls no allocator.  Returns PF_E_ARG when the reduction needs
+ * more than PF_REDUCE_SCRATCH_BYTES (pf_reduce_workspace_bytes(n_cells, K); e.g. K = 20 fits
+ * up to ~2e8 cells).  Every call on one GPU uses that GPU's array: calls that overlap on one
+ * GPU (on different streams, or from several host threads) race; calls ordered on one stream
+ * do not.  Nothing in the array needs to start at zero.
+ */
+#define PF_REDUCE_SCRATCH_BYTES (8u << 20)
+int pf_reduce_noalloc(const void* eta_dev, int32_t dtype, const void* T_dev, const int32_t* argmax0_dev,
+                      const pf_params_t* p, double t, double* out_f64_dev, int64_t* out_i64_dev, void* stream);
 
 /*
  * Batched engine: many independent instances (uneven grids, K, dtype) advanced

@@ -668,7 +668,7 @@ size_t pf_reduce_workspace_bytes(int64_t n_cells, int32_t k) {
 
 namespace {
 
-// Argument checks of pf_reduce (host only, no HIP call):
+// Argument checks shared by pf_reduce and pf_reduce_noalloc (host only, no HIP call):
 // fills the instance description and the workspace size the reduction needs.
 int reduce_check(const void* eta_dev, int32_t dtype, const void* T_dev, const pf_params_t* p,
                  const double* out_f64_dev, const int64_t* out_i64_dev, StepDev& a, size_t& need) {
@@ -724,6 +724,39 @@ int reduce_launch(const StepDev& a, const void* eta_dev, int32_t dtype, const in
 
 }  // namespace
 
+// ---- allocation-free diagnostics (pf_reduce_noalloc)
+// The reduction's partial results live in this fixed array instead of a caller workspace.
+// Each GPU holds its own copy from code-object load until the process ends.  In host code
+// `scratch` names a host-side stand-in, not device memory: kernels get the device address
+// that hipGetSymbolAddress returns for the current device (scratch_dev below).
+static __device__ __align__(256) unsigned char scratch[PF_REDUCE_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices];  // device address of `scratch` per device, looked up once
+
+// Device address of `scratch` on the current device (set up on the first call per device).
+int scratch_dev(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(PF_E_UNSUPPORTED, "pf_reduce_noalloc supports devices 0..63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) return fail(PF_E_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_dev[dev] = ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return PF_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int pf_reduce(const void* eta_dev, int32_t dtype, const void* T_dev, const int32_t* argmax0_dev,
@@ -736,6 +769,22 @@ int pf_reduce(const void* eta_dev, int32_t dtype, const void* T_dev, const int32
   if (!ws_dev) return fail(PF_E_ARG, "NULL workspace");
   if (ws_bytes < need) return fail(PF_E_ARG, "workspace too small");
   return reduce_launch(a, eta_dev, dtype, argmax0_dev, p, t, out_f64_dev, out_i64_dev, ws_dev, stream);
+}
+
+int pf_reduce_noalloc(const void* eta_dev, int32_t dtype, const void* T_dev, const int32_t* argmax0_dev,
+                      const pf_params_t* p, double t, double* out_f64_dev, int64_t* out_i64_dev, void* stream) {
+  StepDev a;
+  size_t need = 0;
+  int rc = reduce_check(eta_dev, dtype, T_dev, p, out_f64_dev, out_i64_dev, a, need);
+  if (rc) return rc;
+  if (need > (size_t)PF_REDUCE_SCRATCH_BYTES)
+    return fail(PF_E_ARG, "pf_reduce_noalloc: the reduction needs " + std::to_string(need) +
+                              " bytes of scratch, more than its fixed " + std::to_string(PF_REDUCE_SCRATCH_BYTES) +
+                              " (use pf_reduce with a workspace)");
+  void* ws = nullptr;
+  rc = scratch_dev(&ws);
+  if (rc) return rc;
+  return reduce_launch(a, eta_dev, dtype, argmax0_dev, p, t, out_f64_dev, out_i64_dev, ws, stream);
 }
 
 }  // extern "C"

@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("PFKIT_LIB") or os.path.join(_HERE, "libpfhip.so")
 
 PF_F64, PF_F32, PF_BF16 = 0, 1, 2
 PF_T_BUFFER, PF_T_ROSENTHAL, PF_T_GAUSSIAN, PF_T_QUENCH = 0, 1, 2, 3
+REDUCE_SCRATCH_BYTES = 8 << 20   # PF_REDUCE_SCRATCH_BYTES: fixed scratch of pf_reduce_noalloc
 PF_PATH_ROW = 10                  # values per row of a batch instance's source table
 ABI_VERSION = 2
 
@@ -83,6 +84,7 @@ SIGNATURES = {
     "pf_voronoi": (ctypes.c_int, [_VP, _I32, _VP, ctypes.POINTER(ParamsT), _VP, _VP]),
     "pf_reduce_workspace_bytes": (_SZ, [_I64, _I32]),
     "pf_reduce": (ctypes.c_int, [_VP, _I32, _VP, _VP, ctypes.POINTER(ParamsT), _D, _VP, _VP, _VP, _SZ, _VP]),
+    "pf_reduce_noalloc": (ctypes.c_int, [_VP, _I32, _VP, _VP, ctypes.POINTER(ParamsT), _D, _VP, _VP, _VP]),
     "pf_batch_create": (_VP, [_I32, _I32, _VP]),
     "pf_batch_destroy": (None, [_VP]),
     "pf_batch_add": (ctypes.c_int, [_VP, ctypes.POINTER(BatchInstT)]),

@@ -73,7 +73,7 @@ class BatchEngine:
             raise _lib.PFError("pf_batch_create: " + self.lib.pf_last_error().decode())
         self.max_instances = max_instances
         self.insts = {}
-        # outputs of the retirement check (pf_reduce), allocated once
+        # outputs of the retirement check (pf_reduce_noalloc), allocated once
         self._red_f = torch.empty(4 + 256, dtype=torch.float64, device=self.device)
         self._red_i = torch.empty(256 + 3, dtype=torch.int64, device=self.device)
         self._red_ws = None
@@ -170,7 +170,8 @@ class BatchEngine:
         return np.frombuffer(sl, dtype=np.int32).copy(), np.frombuffer(tt).copy(), np.frombuffer(pa, np.int32).copy()
 
     def nonfinite(self, slot, parity=None):
-        """Number of inf/nan values in slot's current state (FP64 reduction on the engine's stream)."""
+        """Number of inf/nan values in slot's current state (FP64 reduction on the engine's stream;
+        pf_reduce_noalloc, which allocates nothing, when its fixed scratch holds the reduction)."""
         inst = self.insts[slot]
         eta = self.state(slot, parity)
         p = inst.params
@@ -181,11 +182,15 @@ class BatchEngine:
                 return int((~torch.isfinite(eta)).sum().item())
         Tptr = None  # batch instances evaluate a source descriptor, never a temperature buffer
         need = lib.pf_reduce_workspace_bytes(inst.N, inst.K)
-        if self._red_ws is None or self._red_ws.numel() < need:
-            with torch.cuda.stream(self.stream):
-                self._red_ws = torch.empty(need, dtype=torch.uint8, device=self.device)
-        rc = lib.pf_reduce(eta.data_ptr(), _lib.TORCH_TO_PF[eta.dtype], Tptr, None, p, float(inst.t0),
-                           self._red_f.data_ptr(), self._red_i.data_ptr(), self._red_ws.data_ptr(), need, st)
+        if need <= _lib.REDUCE_SCRATCH_BYTES:
+            rc = lib.pf_reduce_noalloc(eta.data_ptr(), _lib.TORCH_TO_PF[eta.dtype], Tptr, None, p, float(inst.t0),
+                                       self._red_f.data_ptr(), self._red_i.data_ptr(), st)
+        else:
+            if self._red_ws is None or self._red_ws.numel() < need:
+                with torch.cuda.stream(self.stream):
+                    self._red_ws = torch.empty(need, dtype=torch.uint8, device=self.device)
+            rc = lib.pf_reduce(eta.data_ptr(), _lib.TORCH_TO_PF[eta.dtype], Tptr, None, p, float(inst.t0),
+                               self._red_f.data_ptr(), self._red_i.data_ptr(), self._red_ws.data_ptr(), need, st)
         _lib.check(rc, "nonfinite check")
         self.stream.synchronize()
         return int(self._red_i[inst.K + 2].item())

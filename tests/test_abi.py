@@ -82,3 +82,17 @@ def test_argument_validation_without_gpu():
     rc = lib.pf_step(16, 32, _lib.PF_F32, 3, big, None, 0.0, 1.0, 0, 0, None)
     assert rc == -1 and b"2^32 bytes" in lib.pf_last_error()
 
+
+def test_noalloc_reduce_refuses_oversized_input_without_gpu():
+    # pf_reduce_noalloc checks the reduction's scratch need on the host before any HIP call
+    from pfkit.config import make_params
+    from problems import args, gaussian
+    lib = _lib.load()
+    big = make_params(args(256, 256, 128, 256), (1.0, 1.0, 1.0), gaussian(256, 256, 128))
+    assert lib.pf_reduce_workspace_bytes(256 * 256 * 128, 256) > _lib.REDUCE_SCRATCH_BYTES
+    rc = lib.pf_reduce_noalloc(16, _lib.PF_F32, None, None, big, 0.0, 16, 16, None)
+    assert rc == -1 and b"scratch" in lib.pf_last_error()
+    # the c1 diagnostics (256 x 256 x 64 cells, K = 20) fit the fixed scratch
+    assert lib.pf_reduce_workspace_bytes(256 * 256 * 64, 20) <= _lib.REDUCE_SCRATCH_BYTES
+    rc = lib.pf_reduce_noalloc(16, _lib.PF_F32, None, None, big, 0.0, None, 16, None)
+    assert rc == -1 and b"NULL" in lib.pf_last_error()
