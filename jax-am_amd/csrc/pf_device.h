@@ -877,7 +877,7 @@ struct March {
       inv_h2<TC>(a, ihx2, ihy2, ihz2);
       const bool bad = coef_nonfinite(cf);
       constexpr bool SR = SrMode<TS>::value != 0;
-      uint32_t live = 0;  // wave-uniform: bit i set when channel i was updated, not skipped
+      const uint32_t key = SR ? sr_cell_key(sr_plane_key(d.seed, z), gy * a.nx + gx) + (uint32_t)ch0 * kSrChannel : 0u;
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         const LVv vxm = *(const LVv*)(cb + oxm + g * LV);
@@ -892,24 +892,15 @@ struct March {
             // exactly +0 (the full update evaluates to +0 there, finite coefficients given);
             // skip it wave-wide.
             if (__ballot(bad || any_nz(c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l], lo[i], hi[i]))) {
-              // FP32 / BF16 storage: the increment here, rounded stochastically onto eta below
+              // FP32 / BF16 storage: the increment, rounded stochastically onto eta
               res[i] = update_one<TC, ANISO, SR>(cf, B, ihx2, ihy2, ihz2, c[i], vxm.v[l], vxp.v[l], vym.v[l],
                                                  vyp.v[l], lo[i], hi[i], ax + 12 * (ch0 + i));
-              live |= 1u << i;
+              if constexpr (SR) res[i] = sr_round<TS>(c[i], res[i], key + (uint32_t)i * kSrChannel);
             } else {
               res[i] = (TC)0;
             }
           }
         }
-      }
-      if constexpr (SR) {
-        // after the neighbour reads, so that the rounding's temporaries do not add to their registers
-        const uint32_t key = sr_cell_key(sr_plane_key(d.seed, z), gy * a.nx + gx) + (uint32_t)ch0 * kSrChannel;
-#pragma unroll
-        for (int i = 0; i < KH; ++i)
-          if (live & (1u << i)) res[i] = sr_round<TS>(c[i], res[i], key + (uint32_t)i * kSrChannel);
-      } else {
-        (void)live;
       }
       if (__ballot(cf.liquid)) {
 #pragma unroll

@@ -56,7 +56,7 @@ def run_batched(log, n=64, chunk=20, seed=3):
                 classes=sorted({(w.K, w.dtype) for w in insts}))
 
 
-def run_serving(log, rates=(4.0, 8.0, 16.0, 24.0, 32.0), n_per_level=24, chunk=16, budget_gb=2.0,
+def run_serving(log, rates=(4.0, 8.0, 16.0, 24.0, 32.0), n_per_level=24, chunk=16, budget_gb=1.0,
                 sample_every=8):
     """c4: for each arrival rate, Poisson arrivals of n_per_level requests served with
     continuous batching; reports throughput and latency percentiles, and the FP64-mode error
