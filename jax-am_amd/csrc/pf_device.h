@@ -201,15 +201,6 @@ __device__ __forceinline__ float sr_bf16(float x, uint32_t h) {
 template <class TS> struct SrMode { static constexpr int value = 0; };
 template <> struct SrMode<float> { static constexpr int value = 1; };
 template <> struct SrMode<bf16_t> { static constexpr int value = 2; };
-// eta + increment d, stochastically rounded to the storage type TS (returned in FP32)
-template <class TS>
-__device__ __forceinline__ float sr_round(float c, float d, uint32_t h) {
-  if constexpr (SrMode<TS>::value == 1) {
-    return sr_add_f32(c, d, h);
-  } else {
-    return sr_bf16(c + d, h);
-  }
-}
 
 // Cold path: a cell whose FP32 mobility satisfies L(T) * |hstep * stiff| < 1e-6 changes by
 // less than 1e-6 * |eta| per step, so the relative error of its FP32 mobility (hardware
@@ -453,7 +444,7 @@ __device__ __forceinline__ bool any_nz(double c, double a, double b, double d, d
 // there the full evaluation gives q = 1 and ca = w1 exactly, which is what the skip uses.
 // ax: the rotated cube axes of this orientation, axis k at ax[4k .. 4k+2].
 // With coefficients that carry dt (the step's), RHS = true returns the Euler increment
-// eta_new - eta, which the stochastically rounded stores add to eta (sr_round).
+// eta_new - eta, which FP32 storage adds to eta with stochastic rounding (update_sr).
 template <class TC, bool ANISO, bool RHS = false>
 __device__ __forceinline__ TC update_one(const CellCoef<TC>& cf, TC B, TC ihx2, TC ihy2, TC ihz2,
                                          TC c, TC xm, TC xp, TC ym, TC yp, TC zm, TC zp,
@@ -496,6 +487,29 @@ __device__ __forceinline__ TC update_one(const CellCoef<TC>& cf, TC B, TC ihx2, 
   } else {
     const TC r = fmac(-cf.c1, u, c);
     return fmac(ca, lap, r);
+  }
+}
+
+// The Euler update of one channel in the storage type's rounding: FP64 to nearest; FP32 adds the
+// increment to eta with stochastic rounding (sr_add_f32); BF16 rounds the FP32 update
+// stochastically to the BF16 grid (sr_bf16).  h: the channel's random bits.
+template <class TS, bool ANISO>
+__device__ __forceinline__ typename Traits<TS>::TC update_sr(const CellCoef<typename Traits<TS>::TC>& cf,
+                                                             typename Traits<TS>::TC B, typename Traits<TS>::TC ihx2,
+                                                             typename Traits<TS>::TC ihy2, typename Traits<TS>::TC ihz2,
+                                                             typename Traits<TS>::TC c, typename Traits<TS>::TC xm,
+                                                             typename Traits<TS>::TC xp, typename Traits<TS>::TC ym,
+                                                             typename Traits<TS>::TC yp, typename Traits<TS>::TC zm,
+                                                             typename Traits<TS>::TC zp,
+                                                             const typename Traits<TS>::TC* __restrict__ ax, uint32_t h) {
+  using TC = typename Traits<TS>::TC;
+  if constexpr (SrMode<TS>::value == 1) {
+    return sr_add_f32(c, update_one<TC, ANISO, true>(cf, B, ihx2, ihy2, ihz2, c, xm, xp, ym, yp, zm, zp, ax), h);
+  } else if constexpr (SrMode<TS>::value == 2) {
+    return sr_bf16(update_one<TC, ANISO, false>(cf, B, ihx2, ihy2, ihz2, c, xm, xp, ym, yp, zm, zp, ax), h);
+  } else {
+    (void)h;
+    return update_one<TC, ANISO, false>(cf, B, ihx2, ihy2, ihz2, c, xm, xp, ym, yp, zm, zp, ax);
   }
 }
 
@@ -870,14 +884,25 @@ struct March {
         T32 = gsep ? gauss_T32(d.cold, exy32, zt->e32[n]) : cell_temperature_f32(a, d.cold, gx, gy, z, cell);
       const CellCoef<TC> cf = cell_coef_with<TC>(
           a, d.cold, T32,
-          [&] { return gsep ? gauss_T64(a, exy64, zt->e64[n]) : cell_temperature(a, gx, gy, z, cell, d.t_prev); },
+          [&] {
+            // With FP32 arithmetic the FP64 temperature runs only in wavefronts holding a hot cell.
+            // Its column terms do not change with z, but hoisted out of the march they would occupy
+            // (or spill) registers in every wavefront: the opaque copies keep them inside this branch.
+            int gxo = gx, gyo = gy;
+            if constexpr (sizeof(TC) == 4) asm volatile("" : "+v"(gxo), "+v"(gyo));
+            return gsep ? gauss_T64(a, exy64, zt->e64[n]) : cell_temperature(a, gxo, gyo, z, cell, d.t_prev);
+          },
           d.hstep, S);
       const TC B = lap_B<TC>(a);
       TC ihx2, ihy2, ihz2;
       inv_h2<TC>(a, ihx2, ihy2, ihz2);
       const bool bad = coef_nonfinite(cf);
       constexpr bool SR = SrMode<TS>::value != 0;
-      const uint32_t key = SR ? sr_cell_key(sr_plane_key(d.seed, z), gy * a.nx + gx) + (uint32_t)ch0 * kSrChannel : 0u;
+      uint32_t key = SR ? sr_cell_key(sr_plane_key(d.seed, z), gy * a.nx + gx) + (uint32_t)ch0 * kSrChannel : 0u;
+      // FP32 storage: opaque, otherwise the compiler reassociates key + i * kSrChannel and keeps the
+      // plane-invariant sums ch0 * kSrChannel + i * kSrChannel in registers for the whole march, which
+      // spills the lane-pair kernels (BF16 storage has the registers to spare)
+      if constexpr (SrMode<TS>::value == 1) asm volatile("" : "+v"(key));
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         const LVv vxm = *(const LVv*)(cb + oxm + g * LV);
@@ -892,10 +917,8 @@ struct March {
             // exactly +0 (the full update evaluates to +0 there, finite coefficients given);
             // skip it wave-wide.
             if (__ballot(bad || any_nz(c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l], lo[i], hi[i]))) {
-              // FP32 / BF16 storage: the increment, rounded stochastically onto eta
-              res[i] = update_one<TC, ANISO, SR>(cf, B, ihx2, ihy2, ihz2, c[i], vxm.v[l], vxp.v[l], vym.v[l],
-                                                 vyp.v[l], lo[i], hi[i], ax + 12 * (ch0 + i));
-              if constexpr (SR) res[i] = sr_round<TS>(c[i], res[i], key + (uint32_t)i * kSrChannel);
+              res[i] = update_sr<TS, ANISO>(cf, B, ihx2, ihy2, ihz2, c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l],
+                                            lo[i], hi[i], ax + 12 * (ch0 + i), key + (uint32_t)i * kSrChannel);
             } else {
               res[i] = (TC)0;
             }
@@ -1074,7 +1097,6 @@ __device__ __forceinline__ void generic_cell(const StepDev& a, const StepDyn& dy
   inv_h2<TC>(a, ihx2, ihy2, ihz2);
   const TC* __restrict__ axes = (const TC*)a.axes;
   const bool bad = coef_nonfinite(cf);
-  constexpr bool SR = !RHS && SrMode<TS>::value != 0;
   const uint32_t key = sr_cell_key(sr_plane_key(dy.seed, iz), (int)(cell - (long long)iz * nx * ny));
   for (int i = 0; i < K; ++i) {
     TC ax[12];
@@ -1089,8 +1111,10 @@ __device__ __forceinline__ void generic_cell(const StepDev& a, const StepDyn& dy
              yp = to_tc(in[e + oyp]), zm = to_tc(in[e + ozm]), zp = to_tc(in[e + ozp]);
     TC v;
     if (RHS || __ballot(bad || any_nz(c, xm, xp, ym, yp, zm, zp))) {
-      v = update_one<TC, ANISO, RHS || SR>(cf, B, ihx2, ihy2, ihz2, c, xm, xp, ym, yp, zm, zp, ax);
-      if constexpr (SR) v = sr_round<TS>(c, v, key + (uint32_t)i * kSrChannel);
+      if constexpr (RHS)
+        v = update_one<TC, ANISO, true>(cf, B, ihx2, ihy2, ihz2, c, xm, xp, ym, yp, zm, zp, ax);
+      else
+        v = update_sr<TS, ANISO>(cf, B, ihx2, ihy2, ihz2, c, xm, xp, ym, yp, zm, zp, ax, key + (uint32_t)i * kSrChannel);
     } else {
       v = (TC)0;
     }
