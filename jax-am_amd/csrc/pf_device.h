@@ -186,7 +186,9 @@ __device__ __forceinline__ float sr_add_f32(float c, float d, uint32_t h) {
   const float s = c + d;
   const float e = d - (s - c);
   const float u = __uint_as_float(__float_as_uint(s) & 0x7f800000U) * 0x1p-23f;
-  const float thr = (float)(h + __float_as_uint(d)) * (u * 0x1p-32f);
+  // r uniform in [1, 2) from 23 random bits; thr = (r - 1) u is uniform in [0, u)
+  const float r = __uint_as_float(((h + __float_as_uint(d)) & 0x007fffffU) | 0x3f800000U);
+  const float thr = fmac(r, u, -u);
   return fabsf(e) > thr ? s + copysignf(u, e) : s;
 }
 // x rounded stochastically to the BF16 grid (returned as the FP32 value of that BF16): 16
@@ -426,14 +428,17 @@ __device__ __forceinline__ CellCoef<TC> cell_coef(const StepDev& a, const ColdPa
 template <class TC>
 __device__ __forceinline__ bool coef_nonfinite(const CellCoef<TC>& cf) { return !isfinite(cf.A + cf.c1 + cf.c2); }
 
-// Bitwise "non-zero" test of a channel's 7-point stencil (a -0.0 counts as non-zero).
-__device__ __forceinline__ bool any_nz(float c, float a, float b, float d, float e, float f, float g) {
+// Bitwise "non-zero" test of a channel's 7-point stencil (a -0.0 counts as non-zero), or of `keep`
+// (1 in a cell with non-finite coefficients, which must not skip; 0 otherwise).  One OR chain, so the
+// ballot on it is a single compare.
+__device__ __forceinline__ bool any_nz(float c, float a, float b, float d, float e, float f, float g, uint32_t keep) {
   return (__float_as_uint(c) | __float_as_uint(a) | __float_as_uint(b) | __float_as_uint(d) | __float_as_uint(e) |
-          __float_as_uint(f) | __float_as_uint(g)) != 0u;
+          __float_as_uint(f) | __float_as_uint(g) | keep) != 0u;
 }
-__device__ __forceinline__ bool any_nz(double c, double a, double b, double d, double e, double f, double g) {
+__device__ __forceinline__ bool any_nz(double c, double a, double b, double d, double e, double f, double g,
+                                       uint32_t keep) {
   return (__double_as_longlong(c) | __double_as_longlong(a) | __double_as_longlong(b) | __double_as_longlong(d) |
-          __double_as_longlong(e) | __double_as_longlong(f) | __double_as_longlong(g)) != 0;
+          __double_as_longlong(e) | __double_as_longlong(f) | __double_as_longlong(g) | (long long)keep) != 0;
 }
 
 // One orientation channel: centre value c, x/y/z neighbours (edge-replicated at the
@@ -896,7 +901,7 @@ struct March {
       const TC B = lap_B<TC>(a);
       TC ihx2, ihy2, ihz2;
       inv_h2<TC>(a, ihx2, ihy2, ihz2);
-      const bool bad = coef_nonfinite(cf);
+      const uint32_t keep = coef_nonfinite(cf) ? 1u : 0u;
       constexpr bool SR = SrMode<TS>::value != 0;
       uint32_t key = SR ? sr_cell_key(sr_plane_key(d.seed, z), gy * a.nx + gx) + (uint32_t)ch0 * kSrChannel : 0u;
       // FP32 storage: opaque, otherwise the compiler reassociates key + i * kSrChannel and keeps the
@@ -916,7 +921,7 @@ struct March {
             // A channel that is exactly zero on the whole stencil of every lane stays
             // exactly +0 (the full update evaluates to +0 there, finite coefficients given);
             // skip it wave-wide.
-            if (__ballot(bad || any_nz(c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l], lo[i], hi[i]))) {
+            if (__ballot(any_nz(c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l], lo[i], hi[i], keep))) {
               res[i] = update_sr<TS, ANISO>(cf, B, ihx2, ihy2, ihz2, c[i], vxm.v[l], vxp.v[l], vym.v[l], vyp.v[l],
                                             lo[i], hi[i], ax + 12 * (ch0 + i), key + (uint32_t)i * kSrChannel);
             } else {
@@ -1096,7 +1101,7 @@ __device__ __forceinline__ void generic_cell(const StepDev& a, const StepDyn& dy
   TC ihx2, ihy2, ihz2;
   inv_h2<TC>(a, ihx2, ihy2, ihz2);
   const TC* __restrict__ axes = (const TC*)a.axes;
-  const bool bad = coef_nonfinite(cf);
+  const uint32_t keep = coef_nonfinite(cf) ? 1u : 0u;
   const uint32_t key = sr_cell_key(sr_plane_key(dy.seed, iz), (int)(cell - (long long)iz * nx * ny));
   for (int i = 0; i < K; ++i) {
     TC ax[12];
@@ -1110,7 +1115,7 @@ __device__ __forceinline__ void generic_cell(const StepDev& a, const StepDyn& dy
     const TC c = to_tc(in[e]), xm = to_tc(in[e + oxm]), xp = to_tc(in[e + oxp]), ym = to_tc(in[e + oym]),
              yp = to_tc(in[e + oyp]), zm = to_tc(in[e + ozm]), zp = to_tc(in[e + ozp]);
     TC v;
-    if (RHS || __ballot(bad || any_nz(c, xm, xp, ym, yp, zm, zp))) {
+    if (RHS || __ballot(any_nz(c, xm, xp, ym, yp, zm, zp, keep))) {
       if constexpr (RHS)
         v = update_one<TC, ANISO, true>(cf, B, ihx2, ihy2, ihz2, c, xm, xp, ym, yp, zm, zp, ax);
       else
