@@ -64,19 +64,19 @@ def time_steps(solver, eta0, laser, ts, warmup, steps, tile=None, zchunk=None):
     return wall, e0.elapsed_time(e1), a
 
 
-def settle_gpu(wl, seconds=0.3):
-    """Enqueue ~`seconds` of other GPU work (FP64-mode steps of the same grid on their own buffers)
-    without waiting for it, so that the warm-up and timed steps that follow start on a GPU at its
-    working clocks; a few warm-up steps after an idle host phase otherwise time the clock ramp."""
+def settle_gpu(solver, eta0, wl, seconds=0.2):
+    """Enqueue ~`seconds` of the timed kernel itself (steps of the same solver on their own copy of
+    the state) without waiting for it, so that the warm-up and timed steps that follow start with
+    the GPU at the clocks and power state of this kernel: a few warm-up steps after an idle host
+    phase otherwise time the clock ramp, and settling with other work (FP64-mode steps were tried)
+    leaves the first timed steps in that work's power state."""
     import torch
-    from pfkit import PFSolver
-    s64 = PFSolver(wl.pf_args, wl.polycrystal, dtype="float64", clean=False)
-    a = s64.ini_cond()
+    a = eta0.clone()
     b = torch.empty_like(a)
-    ts = wl.ts(4000)
-    n = max(1, int(seconds / 3.3e-4))
-    for i in range(min(n, len(ts) - 1)):
-        s64.state_rhs.step(a, b, wl.laser, ts[i], ts[i + 1], s64.tile, s64.zchunk)
+    n = max(1, int(seconds / 1.5e-4))
+    ts = wl.ts(n)
+    for i in range(n):
+        solver.state_rhs.step(a, b, wl.laser, ts[i], ts[i + 1], solver.tile, solver.zchunk)
         a, b = b, a
     return a, b
 
@@ -280,7 +280,7 @@ def main():
     eta0 = s.ini_cond()
     ts = wl.ts(args.warmup + args.steps)
     log(f"timing {args.steps} FP32 steps after {args.warmup} warm-up steps (tile {s.tile}, zchunk {s.zchunk})")
-    keep = settle_gpu(wl)   # untimed, still running on the GPU when the warm-up steps are enqueued
+    keep = settle_gpu(s, eta0, wl)   # untimed, still running on the GPU when the warm-up steps are enqueued
     wall, ev_ms, _ = time_steps(s, eta0, wl.laser, ts, args.warmup, args.steps)
     del keep
     upd = wl.N * wl.K

@@ -174,22 +174,20 @@ __host__ __device__ inline uint32_t sr_plane_key(uint32_t seed, int z) {
 }
 __device__ __forceinline__ uint32_t sr_cell_key(uint32_t plane_key, int j) { return mix32((uint32_t)j ^ plane_key); }
 constexpr uint32_t kSrChannel = 0x9e3779b9U;
-// c + d rounded stochastically to FP32.  s = RN(c + d) and its residual e = c + d - s (Fast2Sum:
-// exact when |c| >= |d|, which holds for every small update; a close approximation otherwise);
-// s moves by u = ulp(s) towards e with probability |e| / u, so the expected result is c + d.  (When s
-// is a power of two and e points to zero, the float below s lies u/2 away: the move then skips it,
-// which keeps the expectation exact.)  ulp(s) comes from the exponent bits of s (zero for s = 0 and
-// for subnormal s, where the result stays s).
-// The uniform variate is (h + bits of d) / 2^32: h is uniform and independent of d, so the sum
-// is too, and it cannot be formed before d exists.
+// c + d rounded stochastically to FP32.  The sum is formed in FP64 (exact while the exponents of c
+// and d differ by at most 29, as for every small update; rounded at 2^-52 relative otherwise); 29
+// random bits are added below the FP32 mantissa of that FP64 value and cut off, so the FP32
+// magnitude rounds up with probability (cut-off part) / 2^29 and the expected result is c + d.  The
+// FP64 -> FP32 conversion is then exact for normal FP32 results.  Infinities stay infinite (the
+// added bits never carry into the exponent of a zero mantissa) and NaNs stay NaN.  Nine VALU
+// instructions (two conversions, an FP64 add, a 64-bit integer add, a mask, a conversion back and
+// the random bits) against twelve for an FP32 Fast2Sum residual compared with a random threshold.
+// The random bits are (h + bits of d) mod 2^29: h is uniform and independent of d, so the sum is
+// too, and it cannot be formed before d exists (so it holds no register across the channel loop).
 __device__ __forceinline__ float sr_add_f32(float c, float d, uint32_t h) {
-  const float s = c + d;
-  const float e = d - (s - c);
-  const float u = __uint_as_float(__float_as_uint(s) & 0x7f800000U) * 0x1p-23f;
-  // r uniform in [1, 2) from 23 random bits; thr = (r - 1) u is uniform in [0, u)
-  const float r = __uint_as_float(((h + __float_as_uint(d)) & 0x007fffffU) | 0x3f800000U);
-  const float thr = fmac(r, u, -u);
-  return fabsf(e) > thr ? s + copysignf(u, e) : s;
+  const double x = (double)c + (double)d;
+  const uint64_t b = (uint64_t)__double_as_longlong(x) + (uint64_t)((h + __float_as_uint(d)) & 0x1fffffffU);
+  return (float)__longlong_as_double((long long)(b & ~0x1fffffffULL));
 }
 // x rounded stochastically to the BF16 grid (returned as the FP32 value of that BF16): 16
 // random bits (the top half of h + bits of x, as in sr_add_f32) added below the BF16 mantissa,
