@@ -222,7 +222,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="step throughput + roofline + CPU baseline only")
-    ap.add_argument("--only-step", action="store_true", help="only the timed FP32 steps (for profilers)")
+    ap.add_argument("--only-step", action="store_true",
+                    help="only the warm-up and timed FP32 steps, without clock settling (for profilers)")
     ap.add_argument("--only", choices=("bf16", "f64", "batched"), default=None,
                     help="only the timed c1 BF16 or FP64-mode steps, or only the c3 batched run (for profilers)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -280,7 +281,9 @@ def main():
     eta0 = s.ini_cond()
     ts = wl.ts(args.warmup + args.steps)
     log(f"timing {args.steps} FP32 steps after {args.warmup} warm-up steps (tile {s.tile}, zchunk {s.zchunk})")
-    keep = settle_gpu(s, eta0, wl)   # untimed, still running on the GPU when the warm-up steps are enqueued
+    # untimed, still running on the GPU when the warm-up steps are enqueued (profiler runs skip it,
+    # so that every launch of the step kernel they record is a warm-up or timed step)
+    keep = None if args.only_step else settle_gpu(s, eta0, wl)
     wall, ev_ms, _ = time_steps(s, eta0, wl.laser, ts, args.warmup, args.steps)
     del keep
     upd = wl.N * wl.K
