@@ -672,6 +672,19 @@ template <class TS, int K, int TX, int TY, int H> struct Tile {
 template <class TC>
 __device__ __forceinline__ TC sum_sq_halves(TC lo, TC hi) { return lo + hi; }
 
+// The value of the other lane of a lane pair (lanes 2m and 2m + 1): a DPP quad permutation
+// [1, 0, 3, 2] on the VALU, where a shuffle would round-trip through the LDS crossbar
+// (ds_bpermute) on the path to the cell's coefficients.
+__device__ __forceinline__ float pair_swap(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double pair_swap(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Gaussian z-factors of the planes of one march (LDS), for marches of up to kZTab planes.
 constexpr int kZTab = 64;
 struct ZTab {
@@ -878,7 +891,7 @@ struct March {
       TC part = 0;
 #pragma unroll
       for (int i = 0; i < KH; ++i) part = fmac(c[i], c[i], part);
-      const TC oth = __shfl_xor(part, 1, 64);
+      const TC oth = pair_swap(part);
       return ch0 == 0 ? sum_sq_halves(part, oth) : sum_sq_halves(oth, part);
     }
   }
@@ -944,17 +957,25 @@ struct March {
           }
         }
       }
-      if (__ballot(cf.liquid)) {
-#pragma unroll
-        for (int i = 0; i < KH; ++i) res[i] = cf.liquid ? (TC)0 : res[i];
-      }
       if constexpr (!G::RING3) {
 #if PF_STORE_MODE == 0
         TS* o = (TS*)d.out + cell * K + ch0;
 #else
         TS* o = stage + cl * K + ch0;
 #endif
-        store_cell(o, res);
+        // the store on each side of the liquid branch, so that the common side stores the result
+        // registers as they are (one store after the join costs a register copy of every result)
+        if (__ballot(cf.liquid)) {
+          TC rl[KH];
+#pragma unroll
+          for (int i = 0; i < KH; ++i) rl[i] = cf.liquid ? (TC)0 : res[i];
+          store_cell(o, rl);
+        } else {
+          store_cell(o, res);
+        }
+      } else if (__ballot(cf.liquid)) {
+#pragma unroll
+        for (int i = 0; i < KH; ++i) res[i] = cf.liquid ? (TC)0 : res[i];
       }
     }
     if constexpr (G::RING3) {
