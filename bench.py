@@ -44,13 +44,19 @@ def time_steps(solver, eta0, laser, ts, warmup, steps, tile=None, zchunk=None):
     zchunk = solver.zchunk if zchunk is None else zchunk
     a = eta0.clone()
     b = torch.empty_like(a)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # the events' first record creates them (~70 us on the first timing of a process): done here,
+    # before the warm-up steps, that one-time cost stays out of the timed window
+    e0.record(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    e0.elapsed_time(e1)
     i = 0
     for _ in range(warmup):
         solver.state_rhs.step(a, b, laser, ts[i], ts[i + 1], tile, zchunk)
         a, b = b, a
         i += 1
-    stream = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     e0.record(stream)
@@ -79,6 +85,23 @@ def settle_gpu(solver, eta0, wl, seconds=0.2):
         solver.state_rhs.step(a, b, wl.laser, ts[i], ts[i + 1], solver.tile, solver.zchunk)
         a, b = b, a
     return a, b
+
+
+def zchunk_layout(zchunk, nz):
+    """The z-chunk argument in words: 16 | PF_ZCHUNK_TAIL on 64 planes -> "48 planes in chunks of 16,
+    the lowest 16 in chunks of 8" (the rule of set_tiles in pf_abi.hip)."""
+    from pfkit import _lib
+    zc = zchunk & 0xFFFF
+    if zc <= 0 or not zchunk & _lib.ZCHUNK_TAIL:
+        return f"chunks of {zc} planes" if zc > 0 else "default chunks"
+    zc = min(zc, nz)
+    zs = (zc + 1) // 2
+    nbot = (nz // 4 + zs - 1) // zs * zs
+    if nbot > nz - zc:
+        nbot = (nz - zc) // zs * zs
+    if zs >= zc or nbot <= 0:
+        return f"chunks of {zc} planes"
+    return f"{nz - nbot} planes in chunks of {zc}, the lowest {nbot} in chunks of {zs}"
 
 
 def load_traffic(name, tile, zchunk):
@@ -301,6 +324,7 @@ def main():
         "dtype": "f32", "data": "synthetic (seeded Voronoi polycrystal, one-hot start, moving Gaussian laser)",
         "config": {"workload": "c1: 256x256x64 grid, K=20, 500 grains, cubic anisotropy 0.15, FP32",
                    "cells": wl.N, "orientations": wl.K, "tile": s.tile, "zchunk": s.zchunk,
+                   "zchunk_layout": zchunk_layout(s.zchunk, int(wl.pf_args["Nz"])),
                    "dt_us": wl.dt, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

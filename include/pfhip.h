@@ -32,6 +32,12 @@ extern "C" {
 /* values per row of a batch instance's source table (pf_batch_inst_t.path_dev) */
 #define PF_PATH_ROW 10
 
+/* Flag of a zchunk argument (pf_step, pf_batch_inst_t): the lowest quarter of the z-planes (rounded
+ * up to whole chunks, leaving at least one full-length chunk) is marched in chunks of half the
+ * length.  Marches are dispatched from the top of the domain down, so the workgroups that start last
+ * are the short ones and the launch's tail is shorter.  Results do not depend on the chunking. */
+#define PF_ZCHUNK_TAIL 0x10000
+
 /* storage dtypes of eta (and of the optional temperature buffer) */
 enum pf_dtype { PF_F64 = 0, PF_F32 = 1, PF_BF16 = 2 };
 
@@ -113,7 +119,7 @@ int pf_device_count(void);
  *             otherwise; required when p->anisotropy > 0, else may be NULL.
  *   tile: index of the tile shape (pf_tile_count/pf_tile_shape), -1 for the default, or -2 to
  *         force the generic-K kernel (same arithmetic, one thread per cell, no LDS tiling).
- *   zchunk: z-planes per workgroup march (0 = default).
+ *   zchunk: z-planes per workgroup march (0 = default), optionally ORed with PF_ZCHUNK_TAIL.
  */
 int pf_step(const void* eta_in_dev, void* eta_out_dev, int32_t dtype, const void* T_dev,
             const pf_params_t* p, const void* axes_dev, double t_prev, double t_crt,
@@ -227,7 +233,7 @@ typedef struct pf_batch_inst {
   const void* axes_dev;       /* [3, K, 3] like pf_step */
   int32_t dtype;
   int32_t tile;               /* tile shape index (-1 = default) */
-  int32_t zchunk;             /* 0 = default */
+  int32_t zchunk;             /* 0 = default; optionally | PF_ZCHUNK_TAIL */
   int32_t steps;              /* number of Euler steps to run */
   double t0, dt;              /* start time and fixed step */
   pf_params_t p;

@@ -179,13 +179,27 @@ int march_fits(const pf_params_t* p, int dtype) {
   return PF_OK;
 }
 
-void set_tiles(StepDev& a, int tx, int ty, int zc) {
+void set_tiles(StepDev& a, int tx, int ty, int zchunk) {
+  const bool tail = (zchunk & PF_ZCHUNK_TAIL) != 0;
+  int zc = zchunk & ~PF_ZCHUNK_TAIL;
   if (zc <= 0) zc = default_zchunk(a.nz, a.ny, a.nx, tx, ty);
   if (zc > a.nz) zc = a.nz;
   a.zc = zc;
+  a.zs = zc;
+  a.zsplit = 0;
+  if (tail) {
+    // the lowest quarter of the planes in whole chunks of ceil(zc / 2), at least one full chunk above
+    const int zs = (zc + 1) / 2;
+    int nbot = (a.nz / 4 + zs - 1) / zs * zs;
+    if (nbot > a.nz - zc) nbot = (a.nz - zc) / zs * zs;
+    if (zs < zc && nbot > 0) {
+      a.zs = zs;
+      a.zsplit = nbot;
+    }
+  }
   a.tiles_x = (a.nx + tx - 1) / tx;
   a.tiles_y = (a.ny + ty - 1) / ty;
-  a.tiles_z = (a.nz + zc - 1) / zc;
+  a.tiles_z = (a.zsplit > 0 ? (a.zsplit + a.zs - 1) / a.zs : 0) + (a.nz - a.zsplit + zc - 1) / zc;
 }
 
 StepLaunch generic_step(int dtype, bool aniso) {
@@ -542,7 +556,8 @@ int pf_step(const void* eta_in_dev, void* eta_out_dev, int32_t dtype, const void
   const StepEntry* e = tile == -2 ? nullptr : find_entry(dtype, p->num_oris, aniso, tile);
   hipError_t err;
   if (e) {
-    if (zchunk < 0) return fail(PF_E_ARG, "zchunk must be >= 0");
+    if (zchunk < 0 || (zchunk & ~(PF_ZCHUNK_TAIL | 0xffff)) != 0)
+      return fail(PF_E_ARG, "zchunk must be a plane count in [0, 65535], optionally | PF_ZCHUNK_TAIL");
     if (march_fits(p, dtype)) return PF_E_ARG;
     set_tiles(a, e->tx, e->ty, zchunk);
     const long long nt = (long long)a.tiles_x * a.tiles_y * a.tiles_z;

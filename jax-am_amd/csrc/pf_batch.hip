@@ -266,7 +266,8 @@ int pf_batch_add(pf_batch_t* b, const pf_batch_inst_t* inst) {
   if (((uintptr_t)inst->eta_dev[0] | (uintptr_t)inst->eta_dev[1]) & 15)
     return fail(PF_E_ARG, "eta buffers must be 16-byte aligned");
   if (inst->steps < 0) return fail(PF_E_ARG, "steps must be >= 0");
-  if (inst->zchunk < 0) return fail(PF_E_ARG, "zchunk must be >= 0");
+  if (inst->zchunk < 0 || (inst->zchunk & ~(PF_ZCHUNK_TAIL | 0xffff)) != 0)
+    return fail(PF_E_ARG, "zchunk must be a plane count in [0, 65535], optionally | PF_ZCHUNK_TAIL");
   if (inst->path_dev && inst->p.laser.model != PF_T_ROSENTHAL && inst->p.laser.model != PF_T_GAUSSIAN)
     return fail(PF_E_ARG, "a source table needs a Rosenthal or Gaussian source");
   int slot = -1;

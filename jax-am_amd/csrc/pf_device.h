@@ -100,6 +100,7 @@ struct StepDev {
   const void* axes;    // [3][K][3] in TC
   int nz, ny, nx, k;
   int tiles_x, tiles_y, tiles_z, zc;
+  int zs, zsplit;      // planes [0, zsplit) in chunks of zs, [zsplit, nz) in chunks of zc (zsplit 0: no tail)
   int tmodel, t_f64, force_liquid, pad0;
   float ihx2f, ihy2f, ihz2f, Bf;          // FP32 copies of ihx2, ihy2, ihz2, B
   double hx, hy, hz;
@@ -692,6 +693,19 @@ struct ZTab {
   float e32[kZTab];
 };
 
+// First plane and length of z-chunk tile_z (0 = the lowest): the tail [0, zsplit) in chunks of zs,
+// then chunks of zc up to the top (set_tiles).
+__device__ __forceinline__ void zchunk_range(const StepDev& a, int tile_z, int& zb, int& len) {
+  const int nb = a.zsplit > 0 ? (a.zsplit + a.zs - 1) / a.zs : 0;
+  if (tile_z < nb) {
+    zb = tile_z * a.zs;
+    len = min(zb + a.zs, a.zsplit) - zb;
+  } else {
+    zb = a.zsplit + (tile_z - nb) * a.zc;
+    len = min(zb + a.zc, a.nz) - zb;
+  }
+}
+
 // ------------------------------------------------------------------ 2.5-D march of one tile
 // A workgroup owns a TX x TY column of cells (H threads per cell) and marches it upward through
 // the z-planes [zb, zb + len).  Planes (tile + one-cell halo ring, edge-replicated at the domain
@@ -765,8 +779,7 @@ struct March {
     gx = x0 + lx;
     gy = y0 + ly;
     active = (gx < a.nx) && (gy < a.ny);
-    zb = tile_z * a.zc;
-    len = min(zb + a.zc, a.nz) - zb;
+    zchunk_range(a, tile_z, zb, len);
     buf0 = smem;
     buf1 = smem + G::PLANE;
     buf2 = smem + (G::RING3 ? 2 : 1) * G::PLANE;

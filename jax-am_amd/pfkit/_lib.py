@@ -17,6 +17,7 @@ PF_F64, PF_F32, PF_BF16 = 0, 1, 2
 PF_T_BUFFER, PF_T_ROSENTHAL, PF_T_GAUSSIAN, PF_T_QUENCH = 0, 1, 2, 3
 REDUCE_SCRATCH_BYTES = 8 << 20   # PF_REDUCE_SCRATCH_BYTES: fixed scratch of pf_reduce_noalloc
 PF_PATH_ROW = 10                  # values per row of a batch instance's source table
+ZCHUNK_TAIL = 0x10000             # PF_ZCHUNK_TAIL: the lowest quarter of the planes in half-length chunks
 ABI_VERSION = 2
 
 TORCH_TO_PF = {torch.float64: PF_F64, torch.float32: PF_F32, torch.bfloat16: PF_BF16}

@@ -96,6 +96,8 @@ def candidates(pf_dtype, K, Nz):
     lib = _lib.load()
     nt = lib.pf_tile_count(pf_dtype, K)
     zcs = sorted({z for z in (0, 4, 8, 12, 16, 24, 32, Nz) if z <= Nz})
+    # the same chunks with the lowest quarter of the planes in half-length chunks (dispatched last)
+    zcs += [z | _lib.ZCHUNK_TAIL for z in (8, 12, 16, 24) if z < Nz]
     return [(t, z) for t in range(max(nt, 1)) for z in zcs]
 
 

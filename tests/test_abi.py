@@ -37,6 +37,14 @@ def test_version_and_struct_sizes():
                                              ctypes.sizeof(_lib.BatchInstT))
 
 
+def test_header_constants_match_the_bindings():
+    src = open(os.path.join(ROOT, "include", "pfhip.h")).read()
+    defs = dict(re.findall(r"#define\s+(PF\w+)\s+(\S+)", src))
+    assert int(defs["PFHIP_ABI_VERSION"]) == _lib.ABI_VERSION
+    assert int(defs["PF_PATH_ROW"]) == _lib.PF_PATH_ROW
+    assert int(defs["PF_ZCHUNK_TAIL"], 0) == _lib.ZCHUNK_TAIL
+
+
 def test_tile_tables():
     lib = _lib.load()
     for dt in (_lib.PF_F32, _lib.PF_BF16):
@@ -67,6 +75,9 @@ def test_argument_validation_without_gpu():
     rc = lib.pf_step(16, 32, _lib.PF_F32, 3, p, None, 0.0, 1.0, -1, 0, None)
     assert rc == -1 and b"grid" in lib.pf_last_error()
     p.nx = 8
+    # zchunk: a plane count below 2^16, optionally with PF_ZCHUNK_TAIL; other bits are refused
+    rc = lib.pf_step(16, 32, _lib.PF_F32, 3, p, None, 0.0, 1.0, -1, 0x20000 | 4, None)
+    assert rc == -1 and b"zchunk" in lib.pf_last_error()
     p.anisotropy = 0.1
     rc = lib.pf_step(16, 32, _lib.PF_F32, 3, p, None, 0.0, 1.0, -1, 0, None)
     assert rc == -1 and b"grain-direction" in lib.pf_last_error()

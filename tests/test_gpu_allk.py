@@ -59,5 +59,5 @@ def test_every_tiled_instantiation_matches_generic_kernel(K, cuda):
         nt = lib.pf_tile_count(_lib.TORCH_TO_PF[dtype], K)
         assert nt >= 2
         for tile in range(nt):
-            for zc in (0, 4):
+            for zc in (0, 4, 4 | _lib.ZCHUNK_TAIL):
                 assert torch.equal(step(tile, zc), base), (dtype, K, tile, zc)

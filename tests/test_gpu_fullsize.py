@@ -84,7 +84,7 @@ def test_c1_tile_shapes_and_zchunks_agree_bitwise(c1_state):
 
         base = step(-2, 0)  # generic kernel
         for tile in range(lib.pf_tile_count(_lib.TORCH_TO_PF[dtype], wl.K)):
-            for zc in (0, 8, 12, 64):
+            for zc in (0, 8, 12, 64, 10 | _lib.ZCHUNK_TAIL, 16 | _lib.ZCHUNK_TAIL):
                 assert torch.equal(step(tile, zc), base), (dtype, tile, zc)
 
 

@@ -132,7 +132,9 @@ def test_tile_shapes_and_generic_kernel_agree_bitwise(K, dims, cuda):
             assert nt >= 2
             base = _gpu_step(s, eta, L, 0.1, 0.12, dtype, tile=-2)  # generic kernel
             for tile in range(nt):
-                for zc in (0, 1, 3, 7, Nz):
+                # PF_ZCHUNK_TAIL: the lowest planes in half-length chunks (every plane exactly once)
+                T = _lib.ZCHUNK_TAIL
+                for zc in (0, 1, 3, 7, Nz, 2 | T, 3 | T, 7 | T, Nz | T):
                     got = _gpu_step(s, eta, L, 0.1, 0.12, dtype, tile=tile, zchunk=zc)
                     assert torch.equal(got, base), (dtype, tile, zc)
 
