@@ -56,6 +56,22 @@ def test_batched_equals_solo_bitwise(cuda):
     eng.close()
 
 
+def test_batched_zchunk_layouts_equal_solo_bitwise(cuda):
+    # the same instances batched with explicit z-chunks, with and without the tail split
+    from pfkit import _lib
+    wls = _mixed()
+    solo = [_solo(w) for w in wls]
+    for zc in (3, 4 | _lib.ZCHUNK_TAIL, 8 | _lib.ZCHUNK_TAIL):
+        print(f"zchunk {zc}", flush=True)
+        eng = BatchEngine(max_instances=16)
+        slots = [eng.add(w.instance(zchunk=zc)) for w in wls]
+        eng.run_to_completion(chunk=5)
+        _, _, par = eng.status()
+        for w, sl, ref in zip(wls, slots, solo):
+            assert torch.equal(eng.state(sl, par[sl]), ref), (w.name, zc)
+        eng.close()
+
+
 def test_batch_slots_come_and_go(cuda):
     wls = _mixed()
     solo = {w.name: _solo(w) for w in wls}
