@@ -172,10 +172,11 @@ int default_zchunk(int nz, int ny, int nx, int tx, int ty) {
   return zc;
 }
 
-// The tiled kernels (the march) address a z-plane with unsigned 32-bit byte offsets.
+// The tiled kernels (the march) address a z-plane with 32-bit byte offsets and write it
+// back through raw buffer stores whose out-of-range offsets (2^31 and above) are dropped.
 int march_fits(const pf_params_t* p, int dtype) {
-  if ((long long)p->ny * p->nx * p->num_oris * (long long)dtype_size(dtype) >= (1LL << 32))
-    return fail(PF_E_ARG, "the tiled kernels need a z-plane of fewer than 2^32 bytes (tile -2 runs the generic kernel)");
+  if ((long long)p->ny * p->nx * p->num_oris * (long long)dtype_size(dtype) >= (1LL << 31))
+    return fail(PF_E_ARG, "the tiled kernels need a z-plane of fewer than 2^31 bytes (tile -2 runs the generic kernel)");
   return PF_OK;
 }
 

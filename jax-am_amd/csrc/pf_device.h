@@ -549,6 +549,28 @@ template <> __device__ __forceinline__ void inv_h2<double>(const StepDev& a, dou
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// s_waitcnt immediate of gfx9 for vmcnt(0) alone (expcnt and lgkmcnt fields at their maxima)
+constexpr int kWaitVm0 = 0x0F70;
+
+// Raw buffer stores: the fourth descriptor word of an untyped gfx9 buffer, and an offset
+// that the hardware's range check drops (any buffer of the march is under 2^31 bytes).
+constexpr int kRawBufferDword3 = 0x00020000;
+constexpr unsigned kOob = 0x80000000u;
+
+// N bytes from src (N = 1, 2, 4, 8 or 16, aligned to N) to byte offset off of buffer r;
+// NT marks the store non-temporal.
+template <int N, bool NT>
+__device__ __forceinline__ void buffer_store(const void* src, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  constexpr int aux = NT ? 2 : 0;
+  typedef unsigned U2 __attribute__((ext_vector_type(2)));
+  typedef unsigned U4 __attribute__((ext_vector_type(4)));
+  if constexpr (N == 16) __builtin_amdgcn_raw_buffer_store_b128(*(const U4*)src, r, (int)off, 0, aux);
+  else if constexpr (N == 8) __builtin_amdgcn_raw_buffer_store_b64(*(const U2*)src, r, (int)off, 0, aux);
+  else if constexpr (N == 4) __builtin_amdgcn_raw_buffer_store_b32(*(const unsigned*)src, r, (int)off, 0, aux);
+  else if constexpr (N == 2) __builtin_amdgcn_raw_buffer_store_b16(*(const unsigned short*)src, r, (int)off, 0, aux);
+  else __builtin_amdgcn_raw_buffer_store_b8(*(const unsigned char*)src, r, (int)off, 0, aux);
+}
+
 // Workgroup barrier that orders LDS accesses only.  __syncthreads() also releases global
 // memory, which makes every wave wait for all of its outstanding global loads and stores
 // (s_waitcnt vmcnt(0)) at the barrier; the march's prefetched plane and its row stores
@@ -654,6 +676,10 @@ template <class TS, int K, int TX, int TY, int H> struct Tile {
   static constexpr int NCT = TAIL ? PY * PX : 0;  // tail chunks per plane (one per cell)
   static constexpr int CPTT = (NCT + NT - 1) / NT;
   static constexpr int PLANE = PY * PX * KP;     // TC elements per LDS plane
+  // LDS plane stride (TC elements): the plane plus one chunk slot at PLANE, which takes the
+  // hand-over's writes of the lanes past the plane's last chunk (so that every lane writes
+  // every chunk without a per-lane condition); 16-byte aligned
+  static constexpr int PSTR = (PLANE + VEC + 3) & ~3;
   // BF16 lane-pair tiles keep a ring of three LDS planes and store their results straight
   // from registers: one barrier per plane (c2: -3 %).  Every other tile keeps two planes and
   // stages its results in LDS for coalesced row stores, two barriers per plane: FP32 pieces
@@ -664,7 +690,7 @@ template <class TS, int K, int TX, int TY, int H> struct Tile {
   static constexpr int NBUF = RING3 ? 3 : 2;
   static constexpr int STAGE = PF_STORE_MODE != 0 && !RING3 ? NC * K : 8;  // TS elements of output staging
   static constexpr int AXN = 12 * K;                                        // TC elements of the LDS axis table
-  static constexpr size_t LDS_BYTES = NBUF * PLANE * sizeof(TC) + STAGE * sizeof(TS) + AXN * sizeof(TC) + 768;
+  static constexpr size_t LDS_BYTES = NBUF * PSTR * sizeof(TC) + STAGE * sizeof(TS) + AXN * sizeof(TC) + 768;
 };
 
 // The sum of squares S = sum_i eta_i^2 of a cell is formed as two sequential FMA chains, over
@@ -760,7 +786,12 @@ struct March {
   static constexpr int CPR = ROWB / CH;
   static constexpr int NCHO = TY * CPR;
   static constexpr int NJO = (NCHO + G::NT - 1) / G::NT;
-  static constexpr unsigned kNoPiece = ~0u;
+  static constexpr unsigned kNoPiece = kOob;
+  // FP64 tiles write their rows back through buffer stores that every lane issues (see
+  // store_rows): FP64 c1 0.3355 -> 0.3247 ms.  FP32 rows measured slower that way (c1 0.1351 ->
+  // 0.1367 ms), and so did the three-plane ring's per-cell BF16 stores (c2 0.1006 -> 0.1252 ms):
+  // there, waiting for the previous plane's stores at each hand-over is the faster schedule.
+  static constexpr bool BUF_ROWS = PF_STORE_MODE != 0 && !WIDE && sizeof(TC) == 8;
   unsigned opos[PF_STORE_MODE != 0 ? NJO : 1];
   int gx, gy, cl, ch0, own, oxm, oxp, oym, oyp, zb, len, x0, y0;
   bool active, gsep;
@@ -781,8 +812,8 @@ struct March {
     active = (gx < a.nx) && (gy < a.ny);
     zchunk_range(a, tile_z, zb, len);
     buf0 = smem;
-    buf1 = smem + G::PLANE;
-    buf2 = smem + (G::RING3 ? 2 : 1) * G::PLANE;
+    buf1 = smem + G::PSTR;
+    buf2 = smem + (G::RING3 ? 2 : 1) * G::PSTR;
     stage = stage_;
     ax = ax_;
     if constexpr (ANISO) stage_axes<TC, K>((const TC*)a.axes, ax_, tid, G::NT);
@@ -807,7 +838,7 @@ struct March {
       const int sy = clampi(y0 - 1 + r, 0, a.ny - 1);
       const int sx = clampi(x0 - 1 + cc, 0, a.nx - 1);
       src_off[j] = (unsigned)((sy * a.nx + sx) * K + v * VEC) * (unsigned)sizeof(TS);
-      dst_off[j] = (w < G::NCH) ? cs * KP + v * VEC : -1;
+      dst_off[j] = (w < G::NCH) ? cs * KP + v * VEC : G::PLANE;
     }
     if constexpr (TAIL > 0) {
 #pragma unroll
@@ -818,7 +849,7 @@ struct March {
         const int sy = clampi(y0 - 1 + r, 0, a.ny - 1);
         const int sx = clampi(x0 - 1 + cc, 0, a.nx - 1);
         tsrc_off[j] = (unsigned)((sy * a.nx + sx) * K + G::CPC * VEC) * (unsigned)sizeof(TS);
-        tdst_off[j] = (w < G::NCT) ? cs * KP + G::CPC * VEC : -1;
+        tdst_off[j] = (w < G::NCT) ? cs * KP + G::CPC * VEC : G::PLANE;
       }
     }
 #if PF_STORE_MODE != 0
@@ -856,28 +887,30 @@ struct March {
   // LDS chunk w sits at w * VEC when cells are unpadded (KP = K, no tail chunk): the offsets
   // are then immediates off one per-thread base instead of a register per chunk.
   static constexpr bool DLIN = KP == G::CPC * VEC && TAIL == 0;
+  // Every lane writes every chunk: the chunks past the plane's end (lanes of the last pass
+  // beyond it, which loaded a copy of the last chunk) land in the slot at PLANE.  With the
+  // last chunk's write behind a per-lane condition, the compiler waited for everything in
+  // flight (the previous plane's row stores included) before requesting the next plane
+  // (c2 BF16 0.1035 -> 0.1006 ms, FP64 0.3355 -> 0.3318, FP32 0.1355 -> 0.1351).
   __device__ __forceinline__ void store_plane(TC* b) {
 #pragma unroll
     for (int j = 0; j < G::CPT; ++j) {
-      const bool ok = DLIN ? (j + 1 < G::CPT || (int)threadIdx.x + j * G::NT < G::NCH) : dst_off[j] >= 0;
-      if (ok) {
-        TC* dst = DLIN ? b + (int)threadIdx.x * VEC + j * G::NT * VEC : b + dst_off[j];
+      TC* dst = !DLIN                 ? b + dst_off[j]
+                : j * G::NT + G::NT <= G::NCH ? b + (int)threadIdx.x * VEC + j * G::NT * VEC
+                                              : b + min((int)threadIdx.x + j * G::NT, G::NCH) * VEC;
 #pragma unroll
-        for (int q = 0; q < VEC / WV; ++q) {
-          WVv c;
+      for (int q = 0; q < VEC / WV; ++q) {
+        WVv c;
 #pragma unroll
-          for (int e = 0; e < WV; ++e) c.v[e] = to_tc(pre[j].v[q * WV + e]);
-          *(WVv*)(dst + q * WV) = c;
-        }
+        for (int e = 0; e < WV; ++e) c.v[e] = to_tc(pre[j].v[q * WV + e]);
+        *(WVv*)(dst + q * WV) = c;
       }
     }
     if constexpr (TAIL > 0) {
 #pragma unroll
       for (int j = 0; j < G::CPTT; ++j) {
-        if (tdst_off[j] >= 0) {
 #pragma unroll
-          for (int e = 0; e < TAIL; ++e) b[tdst_off[j] + e] = to_tc(pret[j].v[e]);
-        }
+        for (int e = 0; e < TAIL; ++e) b[tdst_off[j] + e] = to_tc(pret[j].v[e]);
       }
     }
   }
@@ -1060,32 +1093,66 @@ struct March {
     }
   }
 
+  // Output plane z as a raw buffer: a store at offset kOob or beyond is dropped by the
+  // hardware's range check (the plane is under 2^31 bytes).
+  __device__ __forceinline__ __amdgpu_buffer_rsrc_t out_plane(int z) const {
+    return __builtin_amdgcn_make_buffer_rsrc((TS*)d.out + (long long)z * plane_elems(), 0,
+                                             (int)(plane_elems() * (long long)sizeof(TS)), kRawBufferDword3);
+  }
+
   // Coalesced write-back of plane z of the tile from the LDS staging rows: consecutive
-  // lanes store consecutive CH-byte pieces of each contiguous (y, z) row segment.
+  // lanes store consecutive CH-byte pieces of each contiguous (y, z) row segment.  BUF_ROWS:
+  // every lane issues one buffer store per pass (pieces never straddle a row's end unless
+  // WIDE), lanes without a piece at an out-of-range offset, so that every wave has the same
+  // stores in flight however the domain's faces cut the tile, and the compiler's waits at the
+  // next hand-over cover the prefetched plane only.  Otherwise lanes without a piece skip
+  // their store, and since the compiler takes the path with the fewest operations in flight,
+  // the next hand-over also waits for this plane's stores.
   __device__ __forceinline__ void store_rows(int z) {
     using CT = typename RawT<CH>::type;
-    using GT = typename std::conditional<WIDE, raw16a2_t, CT>::type;
-    using UT = typename RawT<U>::type;
-    const int rowb = min(TX, a.nx - x0) * CELLB;
-    TS* out = (TS*)d.out + (long long)z * plane_elems();
+    if constexpr (BUF_ROWS) {
+      const __amdgpu_buffer_rsrc_t r = out_plane(z);
 #pragma unroll
-    for (int j = 0; j < NJO; ++j) {
-      if (opos[j] != kNoPiece) {
-        const int w = threadIdx.x + j * G::NT;
-        const int off = (w % CPR) * CH;
-        const char* src = (const char*)stage + w * CH;  // = row (w / CPR) of the staging rows, byte off
-        char* dst = (char*)out + opos[j];
-        if (!WIDE || off + CH <= rowb) {
-          const GT v = *(const CT*)src;
+      for (int j = 0; j < NJO; ++j) {
+        const int w = min((int)threadIdx.x + j * G::NT, NCHO - 1);
+        buffer_store<CH, PF_STORE_MODE == 2>((const char*)stage + w * CH, r, opos[j]);
+      }
+    } else {
+      using GT = typename std::conditional<WIDE, raw16a2_t, CT>::type;
+      using UT = typename RawT<U>::type;
+      const int rowb = min(TX, a.nx - x0) * CELLB;
+      TS* out = (TS*)d.out + (long long)z * plane_elems();
+#pragma unroll
+      for (int j = 0; j < NJO; ++j) {
+        if (opos[j] != kNoPiece) {
+          const int w = threadIdx.x + j * G::NT;
+          const int off = (w % CPR) * CH;
+          const char* src = (const char*)stage + w * CH;  // = row (w / CPR) of the staging rows, byte off
+          char* dst = (char*)out + opos[j];
+          if (!WIDE || off + CH <= rowb) {
+            const GT v = *(const CT*)src;
 #if PF_STORE_MODE == 2
-          __builtin_nontemporal_store(v, (GT*)dst);
+            __builtin_nontemporal_store(v, (GT*)dst);
 #else
-          *(GT*)dst = v;
+            *(GT*)dst = v;
 #endif
-        } else {
-          for (int q = 0; q < rowb - off; q += U) *(UT*)(dst + q) = *(const UT*)(src + q);
+          } else {
+            for (int q = 0; q < rowb - off; q += U) *(UT*)(dst + q) = *(const UT*)(src + q);
+          }
         }
       }
+    }
+  }
+
+  // The stores of one plane's write-back, all dropped (offset kOob): issued after the
+  // prologue's prefetch, they give the z loop's first hand-over the same loads and stores in
+  // flight as every later one, so that the compiler computes one exact set of waits for all.
+  __device__ __forceinline__ void dropped_stores() {
+    if constexpr (BUF_ROWS) {
+      const __amdgpu_buffer_rsrc_t r = out_plane(zb);
+      const unsigned zero[4] __attribute__((aligned(16))) = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int j = 0; j < NJO; ++j) buffer_store<CH, PF_STORE_MODE == 2>(zero, r, kOob + (unsigned)(j * CH));
     }
   }
 
@@ -1115,10 +1182,16 @@ struct March {
     store_plane(buf0);
     load_plane(zstep(z0, 1));
     store_plane(buf1);
+    // nothing of the prologue stays in flight into the z loop (the compiler may issue the
+    // plane z-1 loads after the LDS planes' loads)
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
     lds_barrier();
     read_center(buf0, r1);
     read_center(buf1, r2);
-    if (len > 1) load_plane(zstep(z0, 2));
+    if (len > 1) {
+      load_plane(zstep(z0, 2));
+      dropped_stores();
+    }
     for (int n = 0; n < len; n += 3) {
       if constexpr (G::RING3) {  // plane zb + m lives in buffer m mod 3
         body(n, r0, r1, r2, buf0, buf2);

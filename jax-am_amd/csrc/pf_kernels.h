@@ -58,7 +58,7 @@ __global__ __launch_bounds__(TX* TY* H) __attribute__((amdgpu_waves_per_eu(H + 1
                                                                                                const StepDyn d) {
   using TC = typename Traits<TS>::TC;
   using G = Tile<TS, K, TX, TY, H>;
-  __shared__ __attribute__((aligned(16))) TC smem[G::NBUF * G::PLANE];
+  __shared__ __attribute__((aligned(16))) TC smem[G::NBUF * G::PSTR];
   __shared__ __attribute__((aligned(16))) TS stage[G::STAGE];
   __shared__ __attribute__((aligned(16))) TC ax[ANISO ? G::AXN : 4];
   __shared__ ZTab zt;
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(TX* TY* H) __attribute__((amdgpu_waves_per_eu(2))) 
     const WorkItem* __restrict__ work_all, const int2* __restrict__ cinfo, int cls) {
   using TC = typename Traits<TS>::TC;
   using G = Tile<TS, K, TX, TY, H>;
-  __shared__ __attribute__((aligned(16))) TC smem[G::NBUF * G::PLANE];
+  __shared__ __attribute__((aligned(16))) TC smem[G::NBUF * G::PSTR];
   __shared__ __attribute__((aligned(16))) TS stage[G::STAGE];
   __shared__ __attribute__((aligned(16))) TC ax[ANISO ? G::AXN : 4];
   __shared__ ZTab zt;

@@ -87,11 +87,12 @@ def test_argument_validation_without_gpu():
     assert lib.pf_batch_create(0, 0, None) is None
     rc = lib.pf_step(16, 40, _lib.PF_F32, None, p, None, 0.0, 1.0, -1, 0, None)
     assert rc == -1 and b"aligned" in lib.pf_last_error()
-    # the tiled kernels address a z-plane with 32-bit byte offsets: a tiled step on 8192 x 8192
-    # x 20 FP32 values (5.4 GB) per plane is refused before any launch
-    big = make_params(args(8192, 8192, 1, 20), (1.0, 1.0, 1.0))
+    # the tiled kernels address a z-plane with 32-bit byte offsets and drop out-of-range buffer
+    # stores at 2^31: a tiled step on 6000 x 6000 x 20 FP32 values (2.9 GB) per plane is refused
+    # before any launch
+    big = make_params(args(6000, 6000, 1, 20), (1.0, 1.0, 1.0))
     rc = lib.pf_step(16, 32, _lib.PF_F32, 3, big, None, 0.0, 1.0, 0, 0, None)
-    assert rc == -1 and b"2^32 bytes" in lib.pf_last_error()
+    assert rc == -1 and b"2^31 bytes" in lib.pf_last_error()
 
 
 def test_noalloc_reduce_refuses_oversized_input_without_gpu():
