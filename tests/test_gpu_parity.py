@@ -139,6 +139,34 @@ def test_tile_shapes_and_generic_kernel_agree_bitwise(K, dims, cuda):
                     assert torch.equal(got, base), (dtype, tile, zc)
 
 
+def test_dropped_row_stores_write_nothing_past_the_output(cuda):
+    """FP64 tiles write their rows back through raw buffer stores; lanes without a piece store at an
+    offset 2^31 past the plane's base, which the hardware's range check drops.  With the output at
+    the start of a buffer that reaches past those addresses, every byte after the output keeps its
+    fill value, and the output equals the generic kernel's bitwise (tiles cut in x and y)."""
+    Nx, Ny, Nz, K = 37, 21, 5, 20
+    a = args(Nx, Ny, Nz, K, anisotropy=0.15, force_liquid=True)
+    pc = poly(Nx, Ny, Nz, K)
+    L = gaussian(Nx, Ny, Nz)
+    eta_np = random_eta(Nx * Ny * Nz, K, seed=11)
+    s = _solver(a, pc, torch.float64, cuda)
+    base = _gpu_step(s, eta_np, L, 0.1, 0.12, torch.float64, tile=-2)  # generic kernel
+    n = Nx * Ny * Nz * K
+    fill = 7.25
+    buf = torch.full(((1 << 31) // 8 + (1 << 21),), fill, dtype=torch.float64, device=cuda)  # 2^31 B + 16 MiB
+    out = buf[:n].view(Nx * Ny * Nz, K)
+    eta = torch.as_tensor(eta_np, dtype=torch.float64, device=cuda)
+    lib = _lib.load()
+    for tile in range(lib.pf_tile_count(_lib.PF_F64, K)):
+        for zc in (0, 2 | _lib.ZCHUNK_TAIL):
+            out.fill_(fill)
+            s.state_rhs.step(eta, out, L, 0.1, 0.12, tile, zc)
+            torch.cuda.synchronize()
+            assert torch.equal(out, base), (tile, zc)
+            assert bool((buf[n:] == fill).all()), (tile, zc)
+    del buf
+
+
 def test_sparse_one_hot_trajectory_f32_and_f64(cuda):
     """From the one-hot state most channels are exactly zero: exercises the skip paths."""
     Nx, Ny, Nz, K = 40, 24, 12, 10
