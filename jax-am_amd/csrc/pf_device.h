@@ -298,10 +298,11 @@ __device__ __forceinline__ T load_now(const T* p) {
 // The same object, read through an offset the compiler cannot prove to be zero (an SGPR set by an
 // asm statement).  Fields read through it are loaded (scalar loads) where they are used instead of
 // being hoisted to the kernel's start: the FP64 hot branch reads some 30 uniform FP64 constants, and
-// hoisted they stay live in scalar registers across the whole z-march.  On the c1 FP32 kernel that
-// spilled 105 SGPRs to VGPR lanes, reloaded by ~90 v_readlane per plane on the common path; read
-// here they cost scalar loads inside the branch only (18 SGPRs spilled, 163 -> 150 VGPRs, c1
-// 0.1440 -> 0.1379 ms, BF16 c2 0.1154 -> 0.1114 ms, bitwise the same results).
+// hoisted they stay live in scalar registers across the whole z-march.  On the FP32 K = 20 8x8
+// lane-pair kernel that spilled 105 SGPRs to VGPR lanes, reloaded by ~90 v_readlane per plane on
+// the common path; read here they cost scalar loads inside the branch only (18 SGPRs spilled,
+// 163 -> 150 VGPRs; a 256x256x64, K = 20 step 0.1440 -> 0.1379 ms in FP32, 0.1154 -> 0.1114 ms
+// with BF16 storage; bitwise the same results).
 template <class T>
 __device__ __forceinline__ const T& reload(const T& r) {
   int zero;
@@ -681,11 +682,11 @@ template <class TS, int K, int TX, int TY, int H> struct Tile {
   // every chunk without a per-lane condition); 16-byte aligned
   static constexpr int PSTR = (PLANE + VEC + 3) & ~3;
   // BF16 lane-pair tiles keep a ring of three LDS planes and store their results straight
-  // from registers: one barrier per plane (c2: -3 %).  Every other tile keeps two planes and
-  // stages its results in LDS for coalesced row stores, two barriers per plane: FP32 pieces
-  // stored per lane (8 bytes at an 80-byte cell stride) cost more than the barrier saves
-  // (c1: 0.142 -> 0.182 ms), and for one-lane tiles a third plane would not fit two
-  // workgroups per CU.
+  // from registers: one barrier per plane (256x256x64, K = 20 BF16 step: -3 %).  Every other
+  // tile keeps two planes and stages its results in LDS for coalesced row stores, two barriers
+  // per plane: FP32 pieces stored per lane (8 bytes at an 80-byte cell stride) cost more than
+  // the barrier saves (the same step in FP32: 0.142 -> 0.182 ms), and for one-lane tiles a
+  // third plane would not fit two workgroups per CU.
   static constexpr bool RING3 = H == 2 && sizeof(TS) == 2;
   static constexpr int NBUF = RING3 ? 3 : 2;
   static constexpr int STAGE = PF_STORE_MODE != 0 && !RING3 ? NC * K : 8;  // TS elements of output staging
@@ -788,9 +789,10 @@ struct March {
   static constexpr int NJO = (NCHO + G::NT - 1) / G::NT;
   static constexpr unsigned kNoPiece = kOob;
   // FP64 tiles write their rows back through buffer stores that every lane issues (see
-  // store_rows): FP64 c1 0.3355 -> 0.3247 ms.  FP32 rows measured slower that way (c1 0.1351 ->
-  // 0.1367 ms), and so did the three-plane ring's per-cell BF16 stores (c2 0.1006 -> 0.1252 ms):
-  // there, waiting for the previous plane's stores at each hand-over is the faster schedule.
+  // store_rows): a 256x256x64, K = 20 FP64 step 0.3355 -> 0.3247 ms.  FP32 rows measured slower
+  // that way (the same step in FP32 0.1351 -> 0.1367 ms), and so did the three-plane ring's
+  // per-cell BF16 stores (0.1006 -> 0.1252 ms): there, waiting for the previous plane's stores
+  // at each hand-over is the faster schedule.
   static constexpr bool BUF_ROWS = PF_STORE_MODE != 0 && !WIDE && sizeof(TC) == 8;
   unsigned opos[PF_STORE_MODE != 0 ? NJO : 1];
   int gx, gy, cl, ch0, own, oxm, oxp, oym, oyp, zb, len, x0, y0;
@@ -891,7 +893,7 @@ struct March {
   // beyond it, which loaded a copy of the last chunk) land in the slot at PLANE.  With the
   // last chunk's write behind a per-lane condition, the compiler waited for everything in
   // flight (the previous plane's row stores included) before requesting the next plane
-  // (c2 BF16 0.1035 -> 0.1006 ms, FP64 0.3355 -> 0.3318, FP32 0.1355 -> 0.1351).
+  // (256x256x64, K = 20 step: BF16 0.1035 -> 0.1006 ms, FP64 0.3355 -> 0.3318, FP32 0.1355 -> 0.1351).
   __device__ __forceinline__ void store_plane(TC* b) {
 #pragma unroll
     for (int j = 0; j < G::CPT; ++j) {
@@ -1002,7 +1004,7 @@ struct March {
       // FP32-arithmetic lane pairs of up to 10 channels per lane request the next group's four
       // neighbour vectors before this group's channels run (their branches would otherwise keep the
       // reads behind them): the LDS latency of a group overlaps the previous group's arithmetic
-      // (c1 FP32 -0.3 %, c2 BF16 -1.9 %, for 4-6 more VGPRs).  Everywhere else the extra live
+      // (256x256x64, K = 20 step: FP32 -0.3 %, BF16 -1.9 %, for 4-6 more VGPRs).  Elsewhere the extra live
       // vectors spill: one-lane tiles read 16-byte vectors (16 more VGPRs), FP64 pairs hold 8-byte
       // elements, and larger halves leave no room; there each group is read as it runs.
       if constexpr (H == 2 && sizeof(TC) == 4 && KH <= 10) {
